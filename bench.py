@@ -1,0 +1,189 @@
+"""Benchmark of the MI355X simulation path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload volume_lut]
+
+A step is one frame of the hot path (projection + fused convolution/demodulation)
+over the workload's resident phantom; the timed region holds K frames enqueued
+back to back on the simulator's stream with the IQ written to device memory,
+bracketed by device synchronisation.  Prints one JSON line.
+
+Workload (default): the 3D volume scan quoted "at 1M scatterers" -- 1M fixed
+float32 points in a 6x6x11 cm box, 64x64 = 4096 pyramidal lines over +-30 deg,
+a 256x64x64 float32 LUT beam profile, FFT length 16384, decimation 2.
+
+roofline: the projection kernel's average duration is measured with HIP events
+on the simulator's stream (store_kernel_details) over the same K frames; its
+algorithmic bytes are 16 B per scatterer evaluation (x, y, z, amplitude read
+once per line in the per-line streaming model) times L*N evaluations per launch.
+
+cpu_baseline: the reference's own CPU algorithm (CpuAlgorithm, compiled from the
+reference sources into oracle/_ref) on a bounded sample of the same workload --
+the first lines of the scan against all scatterers -- with all host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_EVAL = {"fixed": 16, "spline": 52}
+
+
+def build_workload(name: str):
+    from openbcsim_amd import workloads as W
+    if name == "volume_lut":
+        return W.volume_lut()
+    if name == "spline_mmode":
+        return W.spline_mmode()
+    if name == "fixed_sector_2d":
+        return W.fixed_sector_2d()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def projection_key(wl) -> str:
+    return "fixed_projection_kernel_ms" if wl.fixed else "spline_projection_kernel_ms"
+
+
+def run_gpu(wl, steps: int, warmup: int):
+    from openbcsim_amd.hip_rt import DeviceBuffer, hip
+    from openbcsim_amd.pyrfsim import RfSimulator
+
+    sim = RfSimulator("gpu")
+    wl.apply(sim)
+    L, S = sim.output_shape()
+    out = DeviceBuffer(L * S * 8)
+    for _ in range(warmup):
+        sim.simulate_lines_device(out.address, L * S)
+    sim.synchronize()
+    hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.simulate_lines_device(out.address, L * S)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    # kernel durations with HIP events on the simulator's stream, same frames again
+    sim.set_parameter("store_kernel_details", "on")
+    kern = {}
+    for _ in range(steps):
+        sim.simulate_lines_device(out.address, L * S)
+        for key in ("fixed_projection_kernel_ms", "spline_projection_kernel_ms", "kernel_convolve_demodulate_ms",
+                    "kernel_memset_ms", "kernel_render_ms"):
+            try:
+                kern.setdefault(key, []).extend(sim.get_debug_data(key))
+            except RuntimeError:
+                pass
+    sim.set_parameter("store_kernel_details", "off")
+    # PCIe-inclusive rate (host output), reported beside the device-resident value
+    sim.simulate_frame()
+    t2 = time.perf_counter()
+    for _ in range(max(1, steps // 2)):
+        sim.simulate_frame()
+    t3 = time.perf_counter()
+    host_ms = (t3 - t2) * 1e3 / max(1, steps // 2)
+    sim.close()
+    out.close()
+    return L, S, ms_per_step, {k: float(np.mean(v)) for k, v in kern.items() if v}, host_ms
+
+
+def cpu_baseline(wl, target_s: float = 12.0):
+    """Reference CPU algorithm on the first lines of the scan (all scatterers)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from backends import Reference, ref_available, Oracle
+    import copy
+
+    cores = os.cpu_count() or 1
+    if ref_available():
+        kind, make = "reference", Reference
+    else:
+        kind, make = "port", Oracle
+
+    def run(n_lines):
+        sub = copy.copy(wl)
+        o, d, l, t, length = wl.scan
+        sub.scan = (o[:n_lines], d[:n_lines], l[:n_lines], t[:n_lines], length)
+        sim = make()
+        sub.apply(sim)
+        t0 = time.perf_counter()
+        sim.simulate_frame()
+        return time.perf_counter() - t0
+
+    probe = max(1, min(cores, wl.num_lines))
+    t_probe = run(probe)
+    per_line = t_probe / probe
+    n = int(max(probe, min(wl.num_lines, (target_s / 3) / max(per_line, 1e-9))))
+    n = max(cores, (n // cores) * cores) if n >= cores else n
+    n = min(n, wl.num_lines)
+    times = [run(n) for _ in range(3)]
+    t = float(np.median(times))
+    return {"value": n / t, "unit": "IQ lines/s", "cores": cores, "kind": kind,
+            "sample": f"first {n} of {wl.num_lines} lines x all {wl.num_scatterers} scatterers, "
+                      f"median of 3 runs ({t:.2f} s each), OpenMP over {cores} host threads",
+            "scatterer_evals_per_s": n * wl.num_scatterers / t}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="volume_lut")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(lines shard across GPUs without communication; see the multi-GPU design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    wl = build_workload(args.workload)
+    L, S, ms, kern, host_ms = run_gpu(wl, args.steps, args.warmup)
+    N = wl.num_scatterers
+    evals = L * N
+    kind = "fixed" if wl.fixed else "spline"
+    pkey = projection_key(wl)
+    proj_ms = kern.get(pkey)
+    roofline = None
+    if proj_ms:
+        algo_bytes = BYTES_PER_EVAL[kind] * evals
+        achieved = algo_bytes / (proj_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": pkey.replace("_ms", ""), "kernel_ms": round(proj_ms, 4)}
+    line = {
+        "metric": "IQ lines/s & scatterer-evals/s at 1M scatterers; serving p50/p99 frame latency",
+        "value": round(L / (ms * 1e-3), 1),
+        "unit": "IQ lines/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded phantom, scan and LUT)",
+        "config": {"workload": wl.name, "lines": L, "scatterers": N, "samples_per_line": S,
+                   "fft_len": wl.fft_len(), "decimation": int(wl.params.get("radial_decimation", 1)),
+                   "profile": wl.profile[0]},
+        "scatterer_evals_per_s": round(evals / (ms * 1e-3), 1),
+        "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+        "pcie_inclusive_lines_per_s": round(L / (host_ms * 1e-3), 1),
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * bcsim_c.h -- C-ABI of the MI355X (gfx950) OpenBCSim "gpu" simulation path.
+ *
+ * Library: libbcsim_hip.so (openbcsim_amd/libbcsim_hip.so after building).
+ *
+ * Each entry point replaces one member of the reference's plugin interface
+ * bcsim::IAlgorithm (src/core/LibBCSim.hpp:42-91) as implemented by
+ * GpuAlgorithm (src/core/algorithm/GpuAlgorithm.cpp); the citation on each
+ * declaration names the reference member it stands for.  Conventions:
+ *
+ *   - Every function returns a bcsim_status; BCSIM_OK is 0.  On failure the
+ *     calling thread's message is available from bcsim_last_error(); the C++
+ *     facade (include/bcsim/LibBCSim.hpp) rethrows it as std::runtime_error,
+ *     which is what the reference throws for the same conditions.
+ *   - Plain pointers and sizes only.  All inputs are copied before the call
+ *     returns; the caller keeps ownership of its buffers.
+ *   - A handle is used from one thread at a time.  Each handle owns one HIP
+ *     stream on its device; device selection is per handle ("gpu_device").
+ *   - Vectors are float triples; arrays of them are [n][3] row-major.
+ *   - IQ output is complex64 interleaved (re, im), laid out [line][sample].
+ */
+#ifndef BCSIM_C_H
+#define BCSIM_C_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define BCSIM_API __attribute__((visibility("default")))
+#else
+#define BCSIM_API
+#endif
+
+typedef struct bcsim_sim bcsim_sim; /* opaque simulator handle */
+typedef int bcsim_status;
+
+#define BCSIM_OK 0
+#define BCSIM_ERR_INVALID_ARGUMENT 1 /* illegal value, shape or key */
+#define BCSIM_ERR_INVALID_STATE 2    /* call order or missing configuration */
+#define BCSIM_ERR_DEVICE 3           /* HIP runtime or kernel launch failure */
+#define BCSIM_ERR_OUT_OF_MEMORY 4    /* device or pinned-host allocation failed */
+#define BCSIM_ERR_CAPACITY 5         /* caller's output buffer is too small */
+#define BCSIM_ERR_INTERNAL 6         /* logic error inside the library */
+
+/* Message of the last failed call on this thread ("" if none). */
+BCSIM_API const char* bcsim_last_error(void);
+
+/* Factory.  Replaces bcsim::Create (src/core/LibBCSim.hpp:97, LibBCSim.cpp:38-48).
+ * sim_type "gpu" creates the HIP implementation.  Any other type fails with
+ * BCSIM_ERR_INVALID_ARGUMENT ("Illegal algorithm type: ..."); the reference's
+ * "cpu" algorithm is not part of this library. */
+BCSIM_API bcsim_status bcsim_create(const char* sim_type, bcsim_sim** out);
+BCSIM_API bcsim_status bcsim_create_simulator(const char* sim_type, bcsim_sim** out); /* alias */
+BCSIM_API bcsim_status bcsim_destroy(bcsim_sim* sim);
+
+/* IAlgorithm::set_parameter (LibBCSim.hpp:50; BaseAlgorithm.cpp:48-87,
+ * GpuAlgorithm.cpp:68-107).  Keys: verbose, sound_speed, noise_amplitude,
+ * use_arc_projection, radial_decimation, phase_delay, gpu_device,
+ * cuda_streams, threads_per_block, store_kernel_details, noise_seed. */
+BCSIM_API bcsim_status bcsim_set_parameter(bcsim_sim* sim, const char* key, const char* value);
+
+/* IAlgorithm::get_parameter (LibBCSim.hpp:53; GpuAlgorithm.cpp:761-773).
+ * Keys: num_cuda_devices, cur_device_name, plus every key accepted by
+ * bcsim_set_parameter.  Writes a NUL-terminated string of at most buf_len bytes. */
+BCSIM_API bcsim_status bcsim_get_parameter(bcsim_sim* sim, const char* key, char* buf, size_t buf_len);
+
+/* IAlgorithm::set_excitation (LibBCSim.hpp:71; GpuAlgorithm.cpp:379-408).
+ * samples[n] float32; 0 <= center_index < n; fs and demod_freq in Hz. */
+BCSIM_API bcsim_status bcsim_set_excitation(bcsim_sim* sim, const float* samples, int n,
+                                            int center_index, float fs, float demod_freq);
+
+/* IAlgorithm::set_scan_sequence (LibBCSim.hpp:68; GpuAlgorithm.cpp:411-451) with
+ * the Scanline/ScanSequence data (ScanSequence.hpp:39-123).  Must follow
+ * set_excitation.  Each line is validated like the Scanline constructor
+ * (ScanSequence.cpp:37-70: orthonormal to 1e-4); the elevational direction is
+ * lateral x direction.  timestamps[num_lines]; line_length in metres. */
+BCSIM_API bcsim_status bcsim_set_scan_sequence(bcsim_sim* sim, int num_lines, const float* origins,
+                                               const float* directions, const float* lateral_dirs,
+                                               const float* timestamps, float line_length,
+                                               int all_timestamps_equal);
+
+/* IAlgorithm::clear_fixed_scatterers / add_fixed_scatterers (LibBCSim.hpp:56-59;
+ * GpuAlgorithm.cpp:607-614, GpuScatterers.cpp:42-46).  xyza is [n][4] float32
+ * (x, y, z, amplitude), the layout of pyrfsim's add_fixed_scatterers
+ * (src/python/PythonInterface.cpp:77-94).  set_* = clear_* then add_*. */
+BCSIM_API bcsim_status bcsim_clear_fixed_scatterers(bcsim_sim* sim);
+BCSIM_API bcsim_status bcsim_add_fixed_scatterers(bcsim_sim* sim, const float* xyza, size_t n);
+BCSIM_API bcsim_status bcsim_set_fixed_scatterers(bcsim_sim* sim, const float* xyza, size_t n);
+
+/* IAlgorithm::clear_spline_scatterers / add_spline_scatterers (LibBCSim.hpp:62-65;
+ * GpuScatterers.cpp:207-272; SplineScatterers BCSimConfig.hpp:126-165).
+ * degree <= 4; knots[n_knots] with n_knots == num_cp + degree + 1;
+ * control_points [n][num_cp][3]; amplitudes [n]. */
+BCSIM_API bcsim_status bcsim_clear_spline_scatterers(bcsim_sim* sim);
+BCSIM_API bcsim_status bcsim_add_spline_scatterers(bcsim_sim* sim, int degree, const float* knots,
+                                                   int n_knots, const float* control_points,
+                                                   const float* amplitudes, size_t n, int num_cp);
+BCSIM_API bcsim_status bcsim_set_spline_scatterers(bcsim_sim* sim, int degree, const float* knots,
+                                                   int n_knots, const float* control_points,
+                                                   const float* amplitudes, size_t n, int num_cp);
+
+/* IAlgorithm::set_analytical_profile with a GaussianBeamProfile
+ * (LibBCSim.hpp:74; GpuAlgorithm.cpp:453-461; BeamProfile.cpp:34-56). */
+BCSIM_API bcsim_status bcsim_set_analytical_profile(bcsim_sim* sim, float sigma_lateral,
+                                                    float sigma_elevational);
+
+/* IAlgorithm::set_lookup_profile with a LUTBeamProfile (LibBCSim.hpp:77;
+ * GpuAlgorithm.cpp:463-521; BeamProfile.cpp:58-137).  samples is
+ * [n_rad][n_lat][n_ele] float32 (index e + n_ele*l + n_lat*n_ele*r,
+ * BeamProfile.hpp:127-129); every n >= 2.  Sampled with the reference CPU's
+ * trilinear rule, zero outside the table. */
+BCSIM_API bcsim_status bcsim_set_lut_profile(bcsim_sim* sim, float r_min, float r_max, float l_min,
+                                             float l_max, float e_min, float e_max,
+                                             const float* samples, int n_rad, int n_lat, int n_ele);
+
+/* Shape of the next simulate_lines output (no reference counterpart; the
+ * reference returns std::vector<std::vector<>>). */
+BCSIM_API bcsim_status bcsim_get_output_shape(bcsim_sim* sim, int* num_lines, int* num_samples);
+
+/* IAlgorithm::simulate_lines (LibBCSim.hpp:81; GpuAlgorithm.cpp:145-377).
+ * Writes num_lines*num_samples complex64 values into out (capacity counted in
+ * complex elements) and returns when they are there. */
+BCSIM_API bcsim_status bcsim_simulate_lines(bcsim_sim* sim, float* out, size_t capacity,
+                                            int* num_lines, int* num_samples);
+
+/* Device-resident variant of simulate_lines: enqueues the frame on the
+ * handle's stream with the IQ written to device memory d_out (complex64,
+ * [line][sample]) and returns without waiting.  bcsim_synchronize waits. */
+BCSIM_API bcsim_status bcsim_simulate_lines_device(bcsim_sim* sim, void* d_out, size_t capacity,
+                                                   int* num_lines, int* num_samples);
+BCSIM_API bcsim_status bcsim_synchronize(bcsim_sim* sim);
+/* The handle's hipStream_t, as void*. */
+BCSIM_API bcsim_status bcsim_get_stream(bcsim_sim* sim, void** stream);
+
+/* IAlgorithm::get_debug_data (LibBCSim.hpp:84; BaseAlgorithm.cpp:89-94).
+ * With store_kernel_details on, keys hold per-launch milliseconds measured with
+ * HIP events on the handle's stream: kernel_memset_ms, fixed_projection_kernel_ms,
+ * spline_projection_kernel_ms, kernel_render_ms, kernel_convolve_demodulate_ms,
+ * kernel_memcpy_ms.  *count receives the number of values; at most capacity are
+ * written.  Unknown keys fail like the reference ("invalid debug data key"). */
+BCSIM_API bcsim_status bcsim_get_debug_data(bcsim_sim* sim, const char* key, double* out,
+                                            size_t capacity, size_t* count);
+
+/* IAlgorithm::get_total_num_scatterers (LibBCSim.hpp:87; GpuAlgorithm.cpp:755-759). */
+BCSIM_API bcsim_status bcsim_get_total_num_scatterers(bcsim_sim* sim, size_t* out);
+
+/* IAlgorithm::set_logger (LibBCSim.hpp:90; ILog BCSimConfig.hpp:57-69).
+ * level is ILog::LogType (0 DEBUG, 1 INFO, 2 WARNING, 3 FATAL); NULL removes it. */
+typedef void (*bcsim_log_fn)(int level, const char* msg, void* user);
+BCSIM_API bcsim_status bcsim_set_logger(bcsim_sim* sim, bcsim_log_fn fn, void* user);
+
+/* Time-projection bins of the last simulated frame, [line][num_rf] complex64.
+ * No reference counterpart (the reference keeps them internal,
+ * GpuAlgorithm.cpp:205-265); exported so the projection stage can be checked
+ * on its own.  With out == NULL only the shape is returned. */
+BCSIM_API bcsim_status bcsim_debug_get_projection(bcsim_sim* sim, float* out, size_t capacity,
+                                                  int* num_lines, int* num_rf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BCSIM_C_H */

@@ -1,0 +1,81 @@
+"""Build libbcsim_hip.so (the C-ABI library) for gfx950 with hipcc, in-tree.
+
+    python -m openbcsim_amd.build
+
+Compile flags:
+  -ffp-contract=off   the radial coordinate and spline position must round like
+                      the reference's float code (they select a bin); the FFT
+                      code re-enables contraction locally with a pragma
+  -munsafe-fp-atomics hardware float atomics (buffers are coarse-grained hipMalloc)
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                      no packed FP32 instructions
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libbcsim_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
+          "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-I", os.path.join(ROOT, "include"), "-fvisibility=hidden"]
+
+SOURCES = ["sim_kernels.hip", "sim_host.cpp"]
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f"command failed ({res.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source and link the shared library; returns its path."""
+    objs = []
+    obj_dir = os.path.join(PKG, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    for src in SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        out = _run(cmd)
+        if verbose and out.strip():
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    """Build the test-only checkers under oracle/ (the FP64 oracle, and the compiled
+    reference CPU path when the reference tree is present)."""
+    oracle = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", oracle, "-B", "oracle"])
+    if os.path.isdir("/root/reference/src/core"):
+        _run(["make", "-s", "-C", oracle, "-B", "ref"])
+
+
+def build_cxx_check() -> str:
+    """Compile tests/cxx/facade_check.cpp (a C++ IAlgorithm user of the facade
+    header) against the built library, for the GPU tests to run."""
+    out = os.path.join(PKG, "_build", "facade_check")
+    _run(["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cxx", "facade_check.cpp"), "-o", out, "-L", PKG, "-lbcsim_hip",
+          "-Wl,-rpath,$ORIGIN/.."])
+    return out
+
+
+if __name__ == "__main__":
+    path = build(verbose="-v" in sys.argv)
+    build_cxx_check()
+    print(path)

@@ -1,0 +1,126 @@
+// Device-visible data layout of the simulation path and the launch interface
+// between the host orchestration (sim_host.cpp) and the kernels (sim_kernels.hip).
+//
+// HBM layout (per simulator handle):
+//   lines        LineGeom[L]                       64 B per line
+//   fixed sets   float4[N] (x, y, z, amplitude)    16 B per scatterer, one load
+//   spline sets  float  cx[M][N], cy[M][N], cz[M][N], amp[N]
+//                (control-point-major, so the p+1 active control points of a
+//                 line are p+1 coalesced streams; the reference's layout idea,
+//                 GpuScatterers.cpp:257)
+//   spline basis SplineLineBasis[L] per spline set  (host-evaluated, float)
+//   bins         float[L][num_rf] (real) or float2[L][num_rf] (phase delay)
+//   H            float2[N_fft]   excitation spectrum * Hilbert mask / N_fft
+//   demod        float2[n_out]   exp(-2 pi i nfd j D)
+//   twiddles     float2[2*T]     two-level table of exp(-2 pi i k / N_fft)
+//   iq           float2[L][n_out]
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace bcsim_hip {
+
+constexpr int kMaxSplineDegree = 4;   // common_definitions.h:2
+constexpr int kMaxFftLen = 16384;     // one line in 128 KiB of LDS
+constexpr int kMinFftLen = 16;
+constexpr int kLdsBytes = 160 * 1024; // gfx950 LDS per workgroup
+
+struct alignas(16) LineGeom {
+    float ox, oy, oz, t;
+    float rx, ry, rz, p0;
+    float lx, ly, lz, p1;
+    float ex, ey, ez, p2;
+};
+
+struct alignas(16) SplineLineBasis {
+    int lo;                          // first control point index of the sum
+    int count;                       // number of terms (degree + 1, or M with sum_all_cs)
+    float b[kMaxSplineDegree + 1];   // basis values for lo .. lo+count-1
+    float pad;
+};
+
+enum ProfileKind { kProfileGaussian = 0, kProfileLut = 1 };
+
+struct ProfileParams {
+    // Gaussian: exp(-(l*l*inv_tsl + e*e*inv_tse)), inv_ts* = 1/(2 sigma^2)
+    float inv_tsl, inv_tse;
+    // LUT (BeamProfile.cpp:79-130): t = (x - x_min) * inv_dx, trilinear, zero outside
+    const float* lut;
+    int nr, nl, ne;
+    float r_min, l_min, e_min;
+    float inv_dr, inv_dl, inv_de;
+};
+
+struct ProjCommon {
+    const LineGeom* lines;
+    int num_lines;
+    int num_rf;
+    // radial bin = floor(r * scale / denom + 0.5) in double (CpuAlgorithm.cpp:75, :161-162);
+    // evaluated as r*scale*inv_denom with an exact-division fallback near half-samples.
+    double bin_scale;
+    double bin_denom;
+    double bin_inv_denom;
+    // sub-sample phase: phase = (bin - true_index) * phase_per_sample, true_index = r*scale/denom
+    float phase_per_sample;
+    ProfileParams prof;
+};
+
+struct FixedSet {
+    const float4* pts;
+    int n;
+};
+
+struct SplineSet {
+    const float* cx;
+    const float* cy;
+    const float* cz;
+    const float* amp;
+    const SplineLineBasis* basis;   // [num_lines]
+    int n;
+    int num_cp;
+};
+
+struct ProjLaunch {
+    int tile_lines;    // lines held in LDS by one workgroup
+    int slices;        // scatterer slices (grid.y); >1 flushes with global atomics
+    int block;         // threads per workgroup
+    int atomic;        // flush with global atomics (several slices or several datasets)
+};
+
+struct ConvParams {
+    const void* bins;       // float[L][num_rf] or float2[L][num_rf]
+    int complex_bins;
+    int num_lines;
+    int num_rf;
+    int fft_len;
+    int log2_len;
+    int twiddle_bits;       // fine table holds 2^twiddle_bits entries
+    const float2* twiddles; // coarse[2^(log2_len - twiddle_bits)] then fine[2^twiddle_bits]
+    const float2* H;        // [fft_len]
+    const float2* demod;    // [n_out]
+    int center;
+    int decimation;
+    int n_out;
+    float2* iq;             // [L][n_out]
+    // additive noise on the bins before convolution (CpuAlgorithm.cpp:340-347)
+    float noise_amplitude;
+    uint32_t noise_seed;
+    uint32_t noise_counter;
+};
+
+// Launchers (sim_kernels.hip).  All enqueue on `stream` and return the launch status.
+hipError_t launch_fill_zero(float* p, size_t n_floats, hipStream_t stream);
+hipError_t launch_project_fixed(const ProjCommon& c, const FixedSet& s, void* bins, bool complex_bins,
+                                bool arc, bool phase, bool lut, const ProjLaunch& cfg,
+                                hipStream_t stream);
+hipError_t launch_project_spline(const ProjCommon& c, const SplineSet& s, void* bins, bool complex_bins,
+                                 bool arc, bool phase, bool lut, const ProjLaunch& cfg,
+                                 hipStream_t stream);
+hipError_t launch_render_spline(const SplineSet& s, const SplineLineBasis& basis, float4* out,
+                                hipStream_t stream);
+hipError_t launch_convolve_demod(const ConvParams& p, hipStream_t stream);
+int conv_block_threads(int fft_len);
+
+}  // namespace bcsim_hip

@@ -1,0 +1,1041 @@
+// Host side of the MI355X simulation path: the "gpu" algorithm behind the C-ABI
+// of include/bcsim_c.h.
+//
+// Simulator plays the role of the reference's GpuAlgorithm
+// (src/core/algorithm/GpuAlgorithm.cpp): it owns the device copies of the
+// phantom, the scan and the beam profile, derives the per-frame tables, and
+// enqueues a frame as
+//     [zero bins] -> projection kernel per dataset -> fused convolve/demod kernel
+// on one HIP stream, with no host round trip inside the frame (the reference
+// issues about four launches and one constant upload per line, plus four
+// device-wide synchronisations, GpuAlgorithm.cpp:189-360).
+//
+// Semantics follow the reference's CPU algorithm where the two reference
+// implementations differ (FFT length next_pow2(num_rf + len - 1), bins limited
+// to [0, num_rf), demodulation phase referenced to the extracted sample,
+// trilinear LUT sampling on grid points); the error behaviour follows
+// GpuAlgorithm (what it validates, and with which message).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bcsim_c.h"
+#include "sim_device.h"
+
+using namespace bcsim_hip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct BcsimError : std::runtime_error {
+    int code;
+    BcsimError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw BcsimError(code, msg); }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? BCSIM_ERR_OUT_OF_MEMORY
+                                                                                      : BCSIM_ERR_DEVICE;
+        fail(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// Device buffer owned by a simulator.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) return;
+        hip_check(hipMalloc(&p, n), "hipMalloc");
+        bytes = n;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct FixedDataset {
+    DevBuf pts;  // float4[n]
+    size_t n = 0;
+};
+
+struct SplineDataset {
+    int degree = 0;
+    int num_cp = 0;
+    size_t n = 0;
+    std::vector<float> knots;
+    DevBuf cx, cy, cz, amp;  // [num_cp][n] x3, [n]
+    DevBuf basis;            // SplineLineBasis[num_lines]
+    DevBuf rendered;         // float4[n], all-timestamps-equal path
+};
+
+// Vector3D<float>::dot evaluated left to right (vector3.hpp:86-88).
+float dot3(const float* a, const float* b) {
+    float t = a[0] * b[0];
+    t = t + a[1] * b[1];
+    t = t + a[2] * b[2];
+    return t;
+}
+
+// bspline_storve::bsplineBasis<float> (bspline.hpp:45-70): the float recursion
+// whose values the reference multiplies with the control points.
+float special_div(float num, float den) {
+    if (std::fabs((double)num) < 1e-6 && std::fabs((double)den) < 1e-6) return 0.0f;
+    return num / den;
+}
+
+float bspline_basis(int j, int p, float x, const std::vector<float>& knots) {
+    if (p == 0) return (x >= knots[j] && x < knots[j + 1]) ? 1.0f : 0.0f;
+    const float left = special_div((x - knots[j]) * bspline_basis(j, p - 1, x, knots), knots[j + p] - knots[j]);
+    const float right = special_div((knots[j + 1 + p] - x) * bspline_basis(j + 1, p - 1, x, knots),
+                                    knots[j + 1 + p] - knots[j + 1]);
+    return left + right;
+}
+
+// bspline_storve::compute_knot_interval (bspline.hpp:75-82)
+int knot_interval(const std::vector<float>& knots, float t) {
+    for (int i = 0; i + 1 < (int)knots.size(); i++)
+        if (t >= knots[i] && t < knots[i + 1]) return i;
+    fail(BCSIM_ERR_INVALID_ARGUMENT, "compute_knot_interval : could not determine knot interval");
+}
+
+bool parse_bool(const std::string& v) {
+    if (v == "on" || v == "true") return true;
+    if (v == "off" || v == "false") return false;
+    fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid boolean value");
+}
+
+int parse_int(const std::string& v) {
+    try {
+        size_t pos = 0;
+        const int x = std::stoi(v, &pos);
+        return x;
+    } catch (...) {
+        fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid integer value: '" + v + "'");
+    }
+}
+
+float parse_float(const std::string& v) {
+    try {
+        return std::stof(v);
+    } catch (...) {
+        fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid number: '" + v + "'");
+    }
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------------------------
+
+struct bcsim_sim {
+    // parameters (BaseAlgorithm.cpp:36-46, GpuAlgorithm.cpp:46-60 defaults)
+    int verbose = 0;
+    float sound_speed = 1540.0f;
+    float noise_amplitude = 0.0f;
+    uint32_t noise_seed = 0;
+    uint32_t noise_counter = 0;
+    bool arc = true;
+    int decimation = 1;
+    bool phase_delay = false;
+    int device = 0;
+    bool can_change_device = true;
+    int num_streams = 2;
+    int threads_per_block = 128;
+    bool store_details = false;
+
+    hipStream_t stream = nullptr;
+    bcsim_log_fn log_fn = nullptr;
+    void* log_user = nullptr;
+
+    // excitation
+    bool have_exc = false;
+    std::vector<float> exc;
+    int center = 0;
+    float fs = 0.0f;
+    float f_demod = 0.0f;
+
+    // scan
+    bool have_scan = false;
+    std::vector<LineGeom> lines;
+    float line_length = 0.0f;
+    bool all_equal = false;
+
+    // phantom
+    std::vector<std::unique_ptr<FixedDataset>> fixed;
+    std::vector<std::unique_ptr<SplineDataset>> splines;
+
+    // profile
+    int profile = -1;
+    float sigma_l = 0.f, sigma_e = 0.f;
+    DevBuf lut;
+    int lut_nr = 0, lut_nl = 0, lut_ne = 0;
+    float lut_r0 = 0, lut_r1 = 0, lut_l0 = 0, lut_l1 = 0, lut_e0 = 0, lut_e1 = 0;
+
+    // derived device tables
+    bool dirty_scan = true, dirty_conv = true, dirty_basis = true;
+    int num_rf = 0, fft_len = 0, log2_len = 0, n_out = 0;
+    DevBuf d_lines, d_H, d_demod, d_tw, d_bins, d_iq;
+
+    std::map<std::string, std::vector<double>> debug;
+
+    ~bcsim_sim() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void log(int level, const std::string& msg) {
+        if (log_fn) log_fn(level, msg.c_str(), log_user);
+    }
+
+    void use_device() {
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        if (!stream) hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        can_change_device = false;
+    }
+
+    // ---------------------------------------------------------------- parameters
+    void set_parameter(const std::string& key, const std::string& value) {
+        if (key == "verbose") {
+            verbose = parse_int(value);
+        } else if (key == "sound_speed") {
+            const float c = parse_float(value);
+            if (c <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal sound speed");
+            sound_speed = c;
+            dirty_scan = dirty_conv = true;
+        } else if (key == "noise_amplitude") {
+            noise_amplitude = parse_float(value);
+        } else if (key == "noise_seed") {
+            noise_seed = (uint32_t)std::stoul(value);
+            noise_counter = 0;
+        } else if (key == "use_arc_projection") {
+            arc = parse_bool(value);
+        } else if (key == "radial_decimation") {
+            const int d = parse_int(value);
+            if (d <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal radial decimation value");
+            decimation = d;
+            dirty_conv = true;
+        } else if (key == "phase_delay") {
+            phase_delay = parse_bool(value);
+        } else if (key == "gpu_device") {
+            if (!can_change_device) fail(BCSIM_ERR_INVALID_STATE, "cannot change CUDA device now");
+            int count = 0;
+            hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+            const int d = parse_int(value);
+            if (d < 0 || d >= count) fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal device number");
+            device = d;
+        } else if (key == "cuda_streams") {
+            const int n = parse_int(value);
+            if (n > 2) fail(BCSIM_ERR_INVALID_ARGUMENT, "number of CUDA streams exceeds MAX_NUM_CUDA_STREAMS");
+            if (n <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "number of CUDA streams must be more than zero");
+            num_streams = n;  // a frame is one stream-ordered launch sequence; kept for compatibility
+        } else if (key == "threads_per_block") {
+            const int n = parse_int(value);
+            if (n <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid number of threads per block");
+            threads_per_block = n;  // launch shapes come from the tuner; kept for compatibility
+        } else if (key == "store_kernel_details") {
+            store_details = parse_bool(value);
+        } else {
+            fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal parameter name: '" + key + "'");
+        }
+    }
+
+    std::string get_parameter(const std::string& key) {
+        if (key == "num_cuda_devices") {
+            int count = 0;
+            hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+            return std::to_string(count);
+        }
+        if (key == "cur_device_name") {
+            hipDeviceProp_t prop;
+            hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+            return std::string(prop.name) + " (" + prop.gcnArchName + ")";
+        }
+        if (key == "verbose") return std::to_string(verbose);
+        if (key == "sound_speed") return std::to_string(sound_speed);
+        if (key == "noise_amplitude") return std::to_string(noise_amplitude);
+        if (key == "noise_seed") return std::to_string(noise_seed);
+        if (key == "use_arc_projection") return arc ? "on" : "off";
+        if (key == "radial_decimation") return std::to_string(decimation);
+        if (key == "phase_delay") return phase_delay ? "on" : "off";
+        if (key == "gpu_device") return std::to_string(device);
+        if (key == "cuda_streams") return std::to_string(num_streams);
+        if (key == "threads_per_block") return std::to_string(threads_per_block);
+        if (key == "store_kernel_details") return store_details ? "on" : "off";
+        fail(BCSIM_ERR_INVALID_ARGUMENT, "Illegal key: " + key);
+    }
+
+    // ---------------------------------------------------------------- configuration
+    void set_excitation(const float* samples, int n, int center_index, float fs_, float fd) {
+        if (!samples || n <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "excitation must have at least one sample");
+        if (center_index < 0 || center_index >= n)
+            fail(BCSIM_ERR_INVALID_ARGUMENT, "excitation center_index must lie inside the excitation");
+        if (!(fs_ > 0)) fail(BCSIM_ERR_INVALID_ARGUMENT, "sampling frequency must be positive");
+        can_change_device = false;
+        exc.assign(samples, samples + n);
+        center = center_index;
+        fs = fs_;
+        f_demod = fd;
+        have_exc = true;
+        dirty_scan = dirty_conv = dirty_basis = true;
+        log(1, "Number of excitation samples: " + std::to_string(n));
+    }
+
+    void set_scan_sequence(int L, const float* o, const float* d, const float* lat, const float* t,
+                           float length, bool equal) {
+        if (!have_exc) fail(BCSIM_ERR_INVALID_STATE, "Excitation must be configured before scan sequence");
+        if (L < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "negative number of scan lines");
+        if (L > 0 && (!o || !d || !lat || !t)) fail(BCSIM_ERR_INVALID_ARGUMENT, "null scan sequence array");
+        std::vector<LineGeom> g(L);
+        for (int i = 0; i < L; i++) {
+            LineGeom& q = g[i];
+            const float* oi = o + 3 * i;
+            const float* di = d + 3 * i;
+            const float* li = lat + 3 * i;
+            q = LineGeom{oi[0], oi[1], oi[2], t[i], di[0], di[1], di[2], 0.f, li[0], li[1], li[2], 0.f,
+                         0.f, 0.f, 0.f, 0.f};
+            // elevational = lateral x direction (ScanSequence.cpp:40, vector3.hpp:96-101)
+            q.ex = li[1] * di[2] - di[1] * li[2];
+            q.ey = di[0] * li[2] - li[0] * di[2];
+            q.ez = li[0] * di[1] - di[0] * li[1];
+            const float e[3] = {q.ex, q.ey, q.ez};
+            const float th = 1e-4f;  // ScanSequence.cpp:52-70
+            bool ok = std::fabs(dot3(di, li)) < th && std::fabs(dot3(li, e)) < th && std::fabs(dot3(e, di)) < th;
+            ok = ok && std::fabs((double)std::sqrt(dot3(di, di)) - 1.0) < th &&
+                 std::fabs((double)std::sqrt(dot3(li, li)) - 1.0) < th &&
+                 std::fabs((double)std::sqrt(dot3(e, e)) - 1.0) < th;
+            if (!ok) fail(BCSIM_ERR_INVALID_ARGUMENT, "Invalid Scanline: Not orthogonal unit vectors (line " +
+                                                           std::to_string(i) + ")");
+        }
+        lines.swap(g);
+        line_length = length;
+        all_equal = equal;
+        have_scan = true;
+        can_change_device = false;
+        dirty_scan = dirty_conv = dirty_basis = true;
+    }
+
+    void clear_fixed() { fixed.clear(); }
+
+    void add_fixed(const float* xyza, size_t n) {
+        if (n > 0 && !xyza) fail(BCSIM_ERR_INVALID_ARGUMENT, "null scatterer array");
+        if (n > (size_t)0x7fffffff) fail(BCSIM_ERR_INVALID_ARGUMENT, "too many scatterers in one dataset");
+        use_device();
+        auto ds = std::make_unique<FixedDataset>();
+        ds->n = n;
+        ds->pts.ensure(n * sizeof(float4));
+        if (n) hip_check(hipMemcpy(ds->pts.p, xyza, n * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
+        fixed.push_back(std::move(ds));
+        if (verbose) log(1, "Number of fixed scatterers: " + std::to_string(total_fixed()));
+    }
+
+    void clear_spline() { splines.clear(); }
+
+    void add_spline(int degree, const float* knots, int n_knots, const float* cps, const float* amps, size_t n,
+                    int num_cp) {
+        if (degree < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "spline degree must be non-negative");
+        if (degree > kMaxSplineDegree) fail(BCSIM_ERR_INVALID_ARGUMENT, "maximum spline degree supported is 4");
+        if (num_cp <= degree) fail(BCSIM_ERR_INVALID_ARGUMENT, "too few spline control points for given degree");
+        if (n_knots != num_cp + degree + 1)
+            fail(BCSIM_ERR_INVALID_ARGUMENT, "mismatch in number of nodes, degree and control points");
+        if (!knots || (n > 0 && (!cps || !amps))) fail(BCSIM_ERR_INVALID_ARGUMENT, "null spline array");
+        if (n > (size_t)0x7fffffff) fail(BCSIM_ERR_INVALID_ARGUMENT, "too many scatterers in one dataset");
+        use_device();
+        auto ds = std::make_unique<SplineDataset>();
+        ds->degree = degree;
+        ds->num_cp = num_cp;
+        ds->n = n;
+        ds->knots.assign(knots, knots + n_knots);
+        // [n][num_cp][3] -> three control-point-major planes [num_cp][n]
+        std::vector<float> px((size_t)num_cp * n), py((size_t)num_cp * n), pz((size_t)num_cp * n);
+        for (size_t s = 0; s < n; s++)
+            for (int i = 0; i < num_cp; i++) {
+                const float* c = cps + (s * num_cp + i) * 3;
+                px[(size_t)i * n + s] = c[0];
+                py[(size_t)i * n + s] = c[1];
+                pz[(size_t)i * n + s] = c[2];
+            }
+        const size_t b = px.size() * sizeof(float);
+        ds->cx.ensure(b);
+        ds->cy.ensure(b);
+        ds->cz.ensure(b);
+        ds->amp.ensure(n * sizeof(float));
+        if (n) {
+            hip_check(hipMemcpy(ds->cx.p, px.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(ds->cy.p, py.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(ds->cz.p, pz.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(ds->amp.p, amps, n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        }
+        splines.push_back(std::move(ds));
+        dirty_basis = true;
+        if (verbose) log(1, "Number of spline scatterers: " + std::to_string(total_spline()));
+    }
+
+    void set_gaussian(float sl, float se) {
+        profile = kProfileGaussian;
+        sigma_l = sl;
+        sigma_e = se;
+    }
+
+    void set_lut(float r0, float r1, float l0, float l1, float e0, float e1, const float* samples, int nr,
+                 int nl, int ne) {
+        // LUTBeamProfile constructor (BeamProfile.cpp:58-72)
+        if (nr <= 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "Too few radial samples");
+        if (nl <= 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "Too few lateral samples");
+        if (ne <= 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "Too few elevational samples");
+        if (!samples) fail(BCSIM_ERR_INVALID_ARGUMENT, "null LUT samples");
+        use_device();
+        const size_t bytes = (size_t)nr * nl * ne * sizeof(float);
+        lut.ensure(bytes);
+        hip_check(hipMemcpy(lut.p, samples, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        lut_nr = nr;
+        lut_nl = nl;
+        lut_ne = ne;
+        lut_r0 = r0; lut_r1 = r1;
+        lut_l0 = l0; lut_l1 = l1;
+        lut_e0 = e0; lut_e1 = e1;
+        profile = kProfileLut;
+    }
+
+    size_t total_fixed() const {
+        size_t s = 0;
+        for (auto& f : fixed) s += f->n;
+        return s;
+    }
+    size_t total_spline() const {
+        size_t s = 0;
+        for (auto& f : splines) s += f->n;
+        return s;
+    }
+
+    // ---------------------------------------------------------------- derived tables
+    int compute_num_rf() const {
+        // compute_num_rf_samples<float> (common_utils.hpp:37-41)
+        const float max_time = (float)(2.0 * (double)line_length / (double)sound_speed);
+        const float prod = fs * max_time;
+        const double v = std::floor((double)prod + 0.5);
+        if (!(v >= 0.0 && v < 1e9)) fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid line length");
+        return (int)v;
+    }
+
+    void require_configured() {
+        if (!have_exc) fail(BCSIM_ERR_INVALID_STATE, "Excitation not configured.");
+        if (!have_scan) fail(BCSIM_ERR_INVALID_STATE, "Scan sequence not configured.");
+    }
+
+    void update_shape() {
+        require_configured();
+        const int nrf = compute_num_rf();
+        if (nrf < 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "line length gives no RF samples");
+        size_t need = (size_t)nrf + exc.size() - 1;
+        size_t N = 1;
+        while (N < need) N <<= 1;  // next_power_of_two (fft.cpp:80-82)
+        if (N > (size_t)kMaxFftLen) {
+            log(3, "num_rf_samples required: " + std::to_string(nrf));
+            fail(BCSIM_ERR_INVALID_ARGUMENT,
+                 "Too many RF samples required: FFT length " + std::to_string(N) + " exceeds " +
+                     std::to_string(kMaxFftLen));
+        }
+        num_rf = nrf;
+        fft_len = (int)N;
+        log2_len = 0;
+        while ((1 << log2_len) < fft_len) log2_len++;
+        n_out = (num_rf + decimation - 1) / decimation;
+    }
+
+    void upload_conv_tables() {
+        const int N = fft_len;
+        // H = mask * FFT_N(excitation) / N in double (BeamConvolver.cpp:77-91, the 1/N of
+        // GpuAlgorithm.cpp:406), by direct summation over the excitation's few samples.
+        std::vector<float2> H(N);
+        const double two_pi = 2.0 * M_PI;
+        for (int k = 0; k < N; k++) {
+            double mask = 0.0;
+            if (N == 1) mask = 1.0;
+            else if (k == 0) mask = 1.0;
+            else if (k < N / 2) mask = 2.0;
+            else if (k == N / 2) mask = 1.0;
+            double re = 0, im = 0;
+            if (mask != 0.0) {
+                for (size_t n = 0; n < exc.size(); n++) {
+                    const long long kn = ((long long)k * (long long)n) % N;
+                    const double a = -two_pi * (double)kn / (double)N;
+                    re += (double)exc[n] * std::cos(a);
+                    im += (double)exc[n] * std::sin(a);
+                }
+            }
+            H[k] = make_float2((float)(mask * re / N), (float)(mask * im / N));
+        }
+        // demodulation: exp(-2 pi i nfd n), n = j*D, with nfd the reference's float quotient
+        // (CpuAlgorithm.cpp:353-358); the cycle count nfd*n is exact in double.
+        std::vector<float2> dem(n_out);
+        const double nfd = (double)(f_demod / fs);
+        for (int j = 0; j < n_out; j++) {
+            double cyc = nfd * (double)((long long)j * decimation);
+            cyc -= std::floor(cyc);
+            const double a = -two_pi * cyc;
+            dem[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        // two-level twiddles exp(-2 pi i k / N): coarse[k >> b] * fine[k & (2^b - 1)]
+        const int b = log2_len / 2;
+        const int nc = N >> b, nf = 1 << b;
+        std::vector<float2> tw(nc + nf);
+        for (int i = 0; i < nc; i++) {
+            const double a = -two_pi * (double)((long long)i << b) / (double)N;
+            tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        for (int i = 0; i < nf; i++) {
+            const double a = -two_pi * (double)i / (double)N;
+            tw[nc + i] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        d_H.ensure(sizeof(float2) * N);
+        d_demod.ensure(sizeof(float2) * std::max(n_out, 1));
+        d_tw.ensure(sizeof(float2) * tw.size());
+        hip_check(hipMemcpy(d_H.p, H.data(), sizeof(float2) * N, hipMemcpyHostToDevice), "hipMemcpy");
+        if (n_out) hip_check(hipMemcpy(d_demod.p, dem.data(), sizeof(float2) * n_out, hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(d_tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+
+    // Per spline dataset and line: knot interval and the p+1 float basis values
+    // (GpuAlgorithm.cpp:657-692 does this per line with a constant-memory upload).
+    void upload_spline_basis() {
+        const int L = (int)lines.size();
+        for (auto& sp : splines) {
+            std::vector<SplineLineBasis> tab(std::max(L, 1));
+            for (int k = 0; k < L; k++) {
+                const float t = lines[k].t;
+                const int mu = knot_interval(sp->knots, t);
+                const int lo = mu - sp->degree, hi = mu;
+                if (lo < 0 || hi >= sp->num_cp)
+                    fail(BCSIM_ERR_INVALID_ARGUMENT, "timestamp outside the span of the spline basis");
+                SplineLineBasis& B = tab[k];
+                std::memset(&B, 0, sizeof(B));
+                B.lo = lo;
+                B.count = hi - lo + 1;
+                for (int i = 0; i < B.count; i++) B.b[i] = bspline_basis(lo + i, sp->degree, t, sp->knots);
+            }
+            sp->basis.ensure(sizeof(SplineLineBasis) * tab.size());
+            hip_check(hipMemcpy(sp->basis.p, tab.data(), sizeof(SplineLineBasis) * tab.size(),
+                                hipMemcpyHostToDevice), "hipMemcpy");
+        }
+    }
+
+    void prepare() {
+        require_configured();
+        if (lines.empty()) fail(BCSIM_ERR_INVALID_STATE, "No scanlines in scansequence");
+        if (profile < 0) fail(BCSIM_ERR_INVALID_STATE, "No beam profile is configured");
+        use_device();
+        if (dirty_scan || dirty_conv) {
+            update_shape();
+            upload_conv_tables();
+            d_lines.ensure(sizeof(LineGeom) * lines.size());
+            hip_check(hipMemcpy(d_lines.p, lines.data(), sizeof(LineGeom) * lines.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+            dirty_scan = dirty_conv = false;
+        }
+        if (dirty_basis) {
+            upload_spline_basis();
+            dirty_basis = false;
+        }
+        const size_t L = lines.size();
+        d_bins.ensure(L * (size_t)num_rf * (phase_delay ? 8 : 4));
+    }
+
+    ProjCommon proj_common(bool spline_rule) const {
+        ProjCommon c{};
+        c.lines = d_lines.as<LineGeom>();
+        c.num_lines = (int)lines.size();
+        c.num_rf = num_rf;
+        if (spline_rule) {
+            // r*2.0/(c*float(1/fs)) (CpuAlgorithm.cpp:161-162)
+            const float dt = (float)(1.0 / (double)fs);
+            c.bin_scale = 2.0;
+            c.bin_denom = (double)(sound_speed * dt);
+        } else {
+            // r*2.0*fs/c (CpuAlgorithm.cpp:75)
+            c.bin_scale = 2.0 * (double)fs;
+            c.bin_denom = (double)sound_speed;
+        }
+        c.bin_inv_denom = 1.0 / c.bin_denom;
+        c.phase_per_sample = (float)(2.0 * M_PI * (double)f_demod / (double)fs);
+        ProfileParams& P = c.prof;
+        if (profile == kProfileGaussian) {
+            P.inv_tsl = (float)(1.0 / (2.0 * (double)sigma_l * (double)sigma_l));
+            P.inv_tse = (float)(1.0 / (2.0 * (double)sigma_e * (double)sigma_e));
+        } else {
+            P.lut = lut.as<float>();
+            P.nr = lut_nr;
+            P.nl = lut_nl;
+            P.ne = lut_ne;
+            P.r_min = lut_r0;
+            P.l_min = lut_l0;
+            P.e_min = lut_e0;
+            // grid spacing as LUTBeamProfile stores it: a float quotient (BeamProfile.cpp:68-70)
+            P.inv_dr = (float)(1.0 / (double)((lut_r1 - lut_r0) / (float)(lut_nr - 1)));
+            P.inv_dl = (float)(1.0 / (double)((lut_l1 - lut_l0) / (float)(lut_nl - 1)));
+            P.inv_de = (float)(1.0 / (double)((lut_e1 - lut_e0) / (float)(lut_ne - 1)));
+        }
+        return c;
+    }
+
+    // Launch shape of the projection: a tile of lines whose bins fit in LDS, and
+    // enough scatterer slices to give the 256 CUs a few workgroups each.
+    ProjLaunch proj_launch(size_t n) const {
+        ProjLaunch cfg{};
+        const size_t per_line = (size_t)num_rf * (phase_delay ? 8 : 4);
+        int tile = (int)std::min<size_t>(8, (size_t)(kLdsBytes - 1024) / per_line);
+        tile = std::max(1, std::min(tile, (int)lines.size()));
+        cfg.tile_lines = tile;
+        const int tiles = ((int)lines.size() + tile - 1) / tile;
+        const int target = 2048;
+        int slices = std::max(1, (target + tiles - 1) / tiles);
+        const int max_slices = std::max<int>(1, (int)((n + 2047) / 2048));
+        cfg.slices = std::min(slices, max_slices);
+        cfg.block = 256;
+        return cfg;
+    }
+
+    // ---------------------------------------------------------------- frame
+    struct Timed {
+        bcsim_sim* s;
+        const char* key;
+        hipEvent_t a = nullptr, b = nullptr;
+        Timed(bcsim_sim* s_, const char* k) : s(s_), key(k) {
+            if (!s->store_details) return;
+            hip_check(hipEventCreate(&a), "hipEventCreate");
+            hip_check(hipEventCreate(&b), "hipEventCreate");
+            hip_check(hipEventRecord(a, s->stream), "hipEventRecord");
+        }
+        void stop() {
+            if (!a) return;
+            hip_check(hipEventRecord(b, s->stream), "hipEventRecord");
+            s->pending.push_back({key, a, b});
+            a = b = nullptr;
+        }
+        ~Timed() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    };
+    struct Pending {
+        std::string key;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+
+    void collect_timings() {
+        for (auto& p : pending) {
+            float ms = 0.f;
+            if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+                debug[p.key].push_back((double)ms);
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        pending.clear();
+    }
+
+    void enqueue_frame(float2* d_out) {
+        prepare();
+        if (store_details) debug.clear();
+        const bool cplx = phase_delay;
+        const size_t L = lines.size();
+        const size_t bin_floats = L * (size_t)num_rf * (cplx ? 2 : 1);
+        float* bins = d_bins.as<float>();
+
+        // Which datasets need an atomic flush: all but a single dataset projected
+        // by one slice per tile.  Zero the bins unless one launch writes them all.
+        const size_t ndatasets = fixed.size() + splines.size();
+        bool need_zero = true;
+        if (ndatasets == 1) {
+            const size_t n = fixed.empty() ? splines[0]->n : fixed[0]->n;
+            need_zero = proj_launch(n).slices > 1 || n == 0;
+        }
+        if (need_zero) {
+            Timed t(this, "kernel_memset_ms");
+            hip_check(launch_fill_zero(bins, bin_floats, stream), "fill kernel");
+            t.stop();
+        }
+        const bool lut_profile = profile == kProfileLut;
+        for (auto& f : fixed) {
+            if (f->n == 0) continue;
+            ProjLaunch cfg = proj_launch(f->n);
+            cfg.atomic = (cfg.slices > 1 || ndatasets > 1) ? 1 : 0;
+            const ProjCommon c = proj_common(false);
+            Timed t(this, "fixed_projection_kernel_ms");
+            hip_check(launch_project_fixed(c, FixedSet{f->pts.as<float4>(), (int)f->n}, bins, cplx, arc,
+                                           phase_delay, lut_profile, cfg, stream),
+                      "fixed projection kernel");
+            t.stop();
+        }
+        for (auto& sp : splines) {
+            if (sp->n == 0) continue;
+            ProjLaunch cfg = proj_launch(sp->n);
+            cfg.atomic = (cfg.slices > 1 || ndatasets > 1) ? 1 : 0;
+            const ProjCommon c = proj_common(true);
+            SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
+                         sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
+            if (all_equal) {
+                // evaluate once at line 0's time, then project as fixed points with the
+                // spline bin rule (GpuAlgorithm.cpp:181-187, GpuScatterers.cpp:48-123)
+                SplineLineBasis B0;
+                host_basis_line0(*sp, B0);
+                sp->rendered.ensure(sp->n * sizeof(float4));
+                {
+                    Timed t(this, "kernel_render_ms");
+                    hip_check(launch_render_spline(ss, B0, sp->rendered.as<float4>(), stream), "render kernel");
+                    t.stop();
+                }
+                Timed t(this, "spline_projection_kernel_ms");
+                hip_check(launch_project_fixed(c, FixedSet{sp->rendered.as<float4>(), (int)sp->n}, bins, cplx,
+                                               arc, phase_delay, lut_profile, cfg, stream),
+                          "spline projection kernel");
+                t.stop();
+            } else {
+                Timed t(this, "spline_projection_kernel_ms");
+                hip_check(launch_project_spline(c, ss, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
+                          "spline projection kernel");
+                t.stop();
+            }
+        }
+        ConvParams p{};
+        p.bins = bins;
+        p.complex_bins = cplx ? 1 : 0;
+        p.num_lines = (int)L;
+        p.num_rf = num_rf;
+        p.fft_len = fft_len;
+        p.log2_len = log2_len;
+        p.twiddle_bits = log2_len / 2;
+        p.twiddles = d_tw.as<float2>();
+        p.H = d_H.as<float2>();
+        p.demod = d_demod.as<float2>();
+        p.center = center;
+        p.decimation = decimation;
+        p.n_out = n_out;
+        p.iq = d_out;
+        p.noise_amplitude = noise_amplitude > 0.0f ? noise_amplitude : 0.0f;
+        p.noise_seed = noise_seed;
+        p.noise_counter = noise_counter++;
+        {
+            Timed t(this, "kernel_convolve_demodulate_ms");
+            hip_check(launch_convolve_demod(p, stream), "convolve/demodulate kernel");
+            t.stop();
+        }
+    }
+
+    void host_basis_line0(const SplineDataset& sp, SplineLineBasis& B) {
+        const float t = lines[0].t;
+        const int mu = knot_interval(sp.knots, t);
+        std::memset(&B, 0, sizeof(B));
+        B.lo = mu - sp.degree;
+        B.count = sp.degree + 1;
+        for (int i = 0; i < B.count; i++) B.b[i] = bspline_basis(B.lo + i, sp.degree, t, sp.knots);
+    }
+
+    void simulate_device(void* d_out, size_t capacity, int* nl, int* ns) {
+        prepare();
+        const size_t need = lines.size() * (size_t)n_out;
+        if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
+        if (nl) *nl = (int)lines.size();
+        if (ns) *ns = n_out;
+        enqueue_frame(static_cast<float2*>(d_out));
+        if (store_details) collect_timings();
+    }
+
+    void simulate_host(float* out, size_t capacity, int* nl, int* ns) {
+        prepare();
+        const size_t need = lines.size() * (size_t)n_out;
+        if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
+        d_iq.ensure(need * sizeof(float2));
+        enqueue_frame(d_iq.as<float2>());
+        {
+            Timed t(this, "kernel_memcpy_ms");
+            hip_check(hipMemcpyAsync(out, d_iq.p, need * sizeof(float2), hipMemcpyDeviceToHost, stream),
+                      "hipMemcpyAsync");
+            t.stop();
+        }
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (store_details) collect_timings();
+        if (nl) *nl = (int)lines.size();
+        if (ns) *ns = n_out;
+    }
+
+    void debug_projection(float* out, size_t capacity, int* nl, int* nrf) {
+        if (!d_bins.p || dirty_scan) fail(BCSIM_ERR_INVALID_STATE, "no simulated frame");
+        const size_t L = lines.size();
+        if (nl) *nl = (int)L;
+        if (nrf) *nrf = num_rf;
+        if (!out) return;  // shape query
+        if (capacity < L * (size_t)num_rf) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (phase_delay) {
+            hip_check(hipMemcpy(out, d_bins.p, L * num_rf * sizeof(float2), hipMemcpyDeviceToHost), "hipMemcpy");
+        } else {
+            std::vector<float> re(L * (size_t)num_rf);
+            hip_check(hipMemcpy(re.data(), d_bins.p, re.size() * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+            for (size_t i = 0; i < re.size(); i++) {
+                out[2 * i] = re[i];
+                out[2 * i + 1] = 0.0f;
+            }
+        }
+        if (nl) *nl = (int)L;
+        if (nrf) *nrf = num_rf;
+    }
+};
+
+// -------------------------------------------------------------------------------------------
+// C-ABI
+
+namespace {
+template <typename F>
+bcsim_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return BCSIM_OK;
+    } catch (const BcsimError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return BCSIM_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return BCSIM_ERR_INTERNAL;
+    }
+}
+
+void need(bcsim_sim* s) {
+    if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null simulator handle");
+}
+}  // namespace
+
+extern "C" {
+
+BCSIM_API const char* bcsim_last_error(void) { return g_last_error.c_str(); }
+
+BCSIM_API bcsim_status bcsim_create(const char* sim_type, bcsim_sim** out) {
+    return guarded([&] {
+        if (!out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output handle");
+        *out = nullptr;
+        const std::string t = sim_type ? sim_type : "";
+        if (t != "gpu") fail(BCSIM_ERR_INVALID_ARGUMENT, "Illegal algorithm type: " + t);
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (count < 1) fail(BCSIM_ERR_DEVICE, "no HIP device available");
+        *out = new bcsim_sim();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_create_simulator(const char* sim_type, bcsim_sim** out) {
+    return bcsim_create(sim_type, out);
+}
+
+BCSIM_API bcsim_status bcsim_destroy(bcsim_sim* sim) {
+    return guarded([&] { delete sim; });
+}
+
+BCSIM_API bcsim_status bcsim_set_parameter(bcsim_sim* sim, const char* key, const char* value) {
+    return guarded([&] {
+        need(sim);
+        if (!key || !value) fail(BCSIM_ERR_INVALID_ARGUMENT, "null key or value");
+        sim->set_parameter(key, value);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_get_parameter(bcsim_sim* sim, const char* key, char* buf, size_t buf_len) {
+    return guarded([&] {
+        need(sim);
+        if (!key || !buf || buf_len == 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "null key or buffer");
+        const std::string v = sim->get_parameter(key);
+        if (v.size() + 1 > buf_len) fail(BCSIM_ERR_CAPACITY, "buffer too small for parameter value");
+        std::memcpy(buf, v.c_str(), v.size() + 1);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_set_excitation(bcsim_sim* sim, const float* samples, int n, int center_index,
+                                            float fs, float demod_freq) {
+    return guarded([&] {
+        need(sim);
+        sim->set_excitation(samples, n, center_index, fs, demod_freq);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_set_scan_sequence(bcsim_sim* sim, int num_lines, const float* origins,
+                                               const float* directions, const float* lateral_dirs,
+                                               const float* timestamps, float line_length,
+                                               int all_timestamps_equal) {
+    return guarded([&] {
+        need(sim);
+        sim->set_scan_sequence(num_lines, origins, directions, lateral_dirs, timestamps, line_length,
+                               all_timestamps_equal != 0);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_clear_fixed_scatterers(bcsim_sim* sim) {
+    return guarded([&] {
+        need(sim);
+        sim->clear_fixed();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_add_fixed_scatterers(bcsim_sim* sim, const float* xyza, size_t n) {
+    return guarded([&] {
+        need(sim);
+        sim->add_fixed(xyza, n);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_set_fixed_scatterers(bcsim_sim* sim, const float* xyza, size_t n) {
+    return guarded([&] {
+        need(sim);
+        sim->clear_fixed();
+        sim->add_fixed(xyza, n);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_clear_spline_scatterers(bcsim_sim* sim) {
+    return guarded([&] {
+        need(sim);
+        sim->clear_spline();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_add_spline_scatterers(bcsim_sim* sim, int degree, const float* knots, int n_knots,
+                                                   const float* control_points, const float* amplitudes,
+                                                   size_t n, int num_cp) {
+    return guarded([&] {
+        need(sim);
+        sim->add_spline(degree, knots, n_knots, control_points, amplitudes, n, num_cp);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_set_spline_scatterers(bcsim_sim* sim, int degree, const float* knots, int n_knots,
+                                                   const float* control_points, const float* amplitudes,
+                                                   size_t n, int num_cp) {
+    return guarded([&] {
+        need(sim);
+        sim->clear_spline();
+        sim->add_spline(degree, knots, n_knots, control_points, amplitudes, n, num_cp);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_set_analytical_profile(bcsim_sim* sim, float sigma_lateral, float sigma_elevational) {
+    return guarded([&] {
+        need(sim);
+        sim->set_gaussian(sigma_lateral, sigma_elevational);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_set_lut_profile(bcsim_sim* sim, float r_min, float r_max, float l_min, float l_max,
+                                             float e_min, float e_max, const float* samples, int n_rad,
+                                             int n_lat, int n_ele) {
+    return guarded([&] {
+        need(sim);
+        sim->set_lut(r_min, r_max, l_min, l_max, e_min, e_max, samples, n_rad, n_lat, n_ele);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_get_output_shape(bcsim_sim* sim, int* num_lines, int* num_samples) {
+    return guarded([&] {
+        need(sim);
+        sim->update_shape();
+        if (num_lines) *num_lines = (int)sim->lines.size();
+        if (num_samples) *num_samples = sim->n_out;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_simulate_lines(bcsim_sim* sim, float* out, size_t capacity, int* num_lines,
+                                            int* num_samples) {
+    return guarded([&] {
+        need(sim);
+        if (!out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output buffer");
+        sim->simulate_host(out, capacity, num_lines, num_samples);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_simulate_lines_device(bcsim_sim* sim, void* d_out, size_t capacity, int* num_lines,
+                                                   int* num_samples) {
+    return guarded([&] {
+        need(sim);
+        if (!d_out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output buffer");
+        sim->simulate_device(d_out, capacity, num_lines, num_samples);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_synchronize(bcsim_sim* sim) {
+    return guarded([&] {
+        need(sim);
+        if (sim->stream) hip_check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+    });
+}
+
+BCSIM_API bcsim_status bcsim_get_stream(bcsim_sim* sim, void** stream) {
+    return guarded([&] {
+        need(sim);
+        if (!stream) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output");
+        sim->use_device();
+        *stream = (void*)sim->stream;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_get_debug_data(bcsim_sim* sim, const char* key, double* out, size_t capacity,
+                                            size_t* count) {
+    return guarded([&] {
+        need(sim);
+        if (!key) fail(BCSIM_ERR_INVALID_ARGUMENT, "null key");
+        auto it = sim->debug.find(key);
+        if (it == sim->debug.end()) fail(BCSIM_ERR_INVALID_ARGUMENT, std::string("invalid debug data key: ") + key);
+        if (count) *count = it->second.size();
+        const size_t n = std::min(capacity, it->second.size());
+        if (out && n) std::memcpy(out, it->second.data(), n * sizeof(double));
+    });
+}
+
+BCSIM_API bcsim_status bcsim_get_total_num_scatterers(bcsim_sim* sim, size_t* out) {
+    return guarded([&] {
+        need(sim);
+        if (!out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output");
+        *out = sim->total_fixed() + sim->total_spline();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_set_logger(bcsim_sim* sim, bcsim_log_fn fn, void* user) {
+    return guarded([&] {
+        need(sim);
+        sim->log_fn = fn;
+        sim->log_user = user;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_debug_get_projection(bcsim_sim* sim, float* out, size_t capacity, int* num_lines,
+                                                  int* num_rf) {
+    return guarded([&] {
+        need(sim);
+        sim->debug_projection(out, capacity, num_lines, num_rf);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,463 @@
+// HIP kernels of the simulation path, written for gfx950 (CDNA4, wave64, 160 KiB LDS).
+//
+//   project_*_kernel     scatterer projection onto scan lines (reference kernels
+//                        FixedAlgKernel, cuda_kernels_fixed.cuh:15-65, and
+//                        SplineAlgKernel, cuda_kernels_spline2.cu:14-79; arithmetic
+//                        of the CPU algorithm, CpuAlgorithm.cpp:50-183).  A workgroup
+//                        holds the bins of a tile of lines in LDS, every scatterer it
+//                        reads is projected onto all lines of the tile, and the tile
+//                        leaves LDS once, by plain stores or by global atomics when
+//                        the scatterers are split over several workgroups.
+//   render_spline_kernel evaluate splines at one time (RenderSplineKernel,
+//                        cuda_kernels_spline1.cu:15-45).
+//   convolve_demod_kernel one line per workgroup, entirely in LDS: forward FFT,
+//                        multiply by the Hilbert-masked excitation spectrum, inverse
+//                        FFT, delay extraction, demodulation and decimation
+//                        (cuFFT + MultiplyFftKernel + cuFFT + DemodulateKernel +
+//                        host extraction in GpuAlgorithm.cpp:280-376; CPU semantics
+//                        BeamConvolver.cpp:64-73, CpuAlgorithm.cpp:349-365).
+//
+// This file is compiled with -ffp-contract=off: the radial coordinate and the
+// spline position must round exactly like the reference's float code because
+// they select a bin.  The FFT code opts back into contraction locally.
+#include <hip/hip_runtime.h>
+
+#include "sim_device.h"
+
+namespace bcsim_hip {
+
+// ------------------------------------------------------------------------------------
+// utilities
+
+__global__ void fill_zero_kernel(float4* p4, size_t n4, float* tail, int n_tail) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        p4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (blockIdx.x == 0 && (int)threadIdx.x < n_tail) tail[threadIdx.x] = 0.f;
+}
+
+hipError_t launch_fill_zero(float* p, size_t n_floats, hipStream_t stream) {
+    if (n_floats == 0) return hipSuccess;
+    const size_t n4 = n_floats / 4;
+    const int tail = (int)(n_floats - n4 * 4);
+    size_t blocks = (n4 + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<float4*>(p), n4, p + n4 * 4, tail);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// projection
+
+// Beam-local coordinates, float, left-to-right dot products (CpuAlgorithm.cpp:58-72).
+struct Local {
+    float r, l, e;
+};
+
+template <bool ARC>
+__device__ __forceinline__ Local local_coords(const LineGeom& L, float px, float py, float pz) {
+    const float dx = px - L.ox;
+    const float dy = py - L.oy;
+    const float dz = pz - L.oz;
+    Local o;
+    o.r = dx * L.rx + dy * L.ry + dz * L.rz;
+    o.l = dx * L.lx + dy * L.ly + dz * L.lz;
+    o.e = dx * L.ex + dy * L.ey + dz * L.ez;
+    if (ARC) o.r = copysignf(__builtin_sqrtf(dx * dx + dy * dy + dz * dz), o.r);
+    return o;
+}
+
+// GaussianBeamProfile::sampleProfile (BeamProfile.cpp:49-51).
+__device__ __forceinline__ float gauss_weight(const ProfileParams& P, float l, float e) {
+    return expf(-(l * l * P.inv_tsl + e * e * P.inv_tse));
+}
+
+// LUTBeamProfile::sampleProfile (BeamProfile.cpp:79-130): truncating indices, zero
+// whenever a neighbour falls outside the table, trilinear in r, then l, then e.
+__device__ __forceinline__ float lut_weight(const ProfileParams& P, float r, float l, float e) {
+    const float tr = (r - P.r_min) * P.inv_dr;
+    const float tl = (l - P.l_min) * P.inv_dl;
+    const float te = (e - P.e_min) * P.inv_de;
+    if (!(tr > -1.0f && tr < (float)(P.nr - 1))) return 0.0f;
+    if (!(tl > -1.0f && tl < (float)(P.nl - 1))) return 0.0f;
+    if (!(te > -1.0f && te < (float)(P.ne - 1))) return 0.0f;
+    const int r0 = (int)tr, r1 = (int)(tr + 1.0f);
+    const int l0 = (int)tl, l1 = (int)(tl + 1.0f);
+    const int e0 = (int)te, e1 = (int)(te + 1.0f);
+    if (r1 >= P.nr || l1 >= P.nl || e1 >= P.ne) return 0.0f;
+    const float fr = tr - (float)r0, fl = tl - (float)l0, fe = te - (float)e0;
+    const int sl = P.ne, sr = P.nl * P.ne;
+    const float* t = P.lut;
+    const float c000 = t[e0 + sl * l0 + sr * r0], c001 = t[e1 + sl * l0 + sr * r0];
+    const float c010 = t[e0 + sl * l1 + sr * r0], c011 = t[e1 + sl * l1 + sr * r0];
+    const float c100 = t[e0 + sl * l0 + sr * r1], c101 = t[e1 + sl * l0 + sr * r1];
+    const float c110 = t[e0 + sl * l1 + sr * r1], c111 = t[e1 + sl * l1 + sr * r1];
+    const float c00 = (1.0f - fr) * c000 + fr * c100;
+    const float c10 = (1.0f - fr) * c010 + fr * c110;
+    const float c01 = (1.0f - fr) * c001 + fr * c101;
+    const float c11 = (1.0f - fr) * c011 + fr * c111;
+    const float c0 = (1.0f - fl) * c00 + fl * c10;
+    const float c1 = (1.0f - fl) * c01 + fl * c11;
+    return (1.0f - fe) * c0 + fe * c1;
+}
+
+// Bin of radial distance r: floor(r*scale/denom + 0.5) evaluated in double as the
+// reference writes it (CpuAlgorithm.cpp:75 fixed, :161-162 spline).  The quotient is
+// first formed with the reciprocal; only when the result lies within 1e-9 of a
+// bin boundary (where the two could round differently) is the exact division
+// done.  Returns -1 for bins outside [0, num_rf) and for non-finite r.
+__device__ __forceinline__ int radial_bin(const ProjCommon& c, float r, double& true_index) {
+    const double q = (double)r * c.bin_scale;
+    true_index = q * c.bin_inv_denom;
+    double t = true_index + 0.5;
+    double fl = floor(t);
+    const double fr = t - fl;
+    if (fr < 1e-9 || fr > 1.0 - 1e-9) {
+        t = q / c.bin_denom + 0.5;
+        fl = floor(t);
+    }
+    if (!(t >= 0.0 && t < (double)c.num_rf)) return -1;
+    return (int)fl;
+}
+
+template <bool ARC, bool PHASE, bool LUT>
+__device__ __forceinline__ void project_point(const ProjCommon& c, const LineGeom& L, float px, float py,
+                                              float pz, float amp, float* lbins) {
+    const Local q = local_coords<ARC>(L, px, py, pz);
+    double true_index;
+    const int bin = radial_bin(c, q.r, true_index);
+    if (bin < 0) return;
+    const float w = (LUT ? lut_weight(c.prof, q.r, q.l, q.e) : gauss_weight(c.prof, q.l, q.e)) * amp;
+    if (w == 0.0f) return;  // adding +-0 leaves the bin unchanged
+    if (PHASE) {
+        // CpuAlgorithm.cpp:85-92: w * exp(i * 2 pi f_demod * (bin - true_index) / fs)
+        const float ph = (float)((double)bin - true_index) * c.phase_per_sample;
+        float s, co;
+        sincosf(ph, &s, &co);
+        atomicAdd(&lbins[2 * bin], w * co);
+        atomicAdd(&lbins[2 * bin + 1], w * s);
+    } else {
+        atomicAdd(&lbins[bin], w);
+    }
+}
+
+__device__ __forceinline__ void zero_lds(float* lds, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = 0.0f;
+}
+
+// Write a tile of LDS bins to HBM: plain stores if this workgroup saw every
+// scatterer of the frame, else an atomic add of the non-zero bins.
+__device__ __forceinline__ void flush_bins(const float* lds, float* out, int n, bool atomic) {
+    if (atomic) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const float v = lds[i];
+            if (v != 0.0f) unsafeAtomicAdd(out + i, v);
+        }
+    } else {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = lds[i];
+    }
+}
+
+template <bool ARC, bool PHASE, bool LUT>
+__global__ __launch_bounds__(256) void project_fixed_kernel(ProjCommon c, FixedSet s, float* bins,
+                                                            int tile_lines, int chunk, int atomic) {
+    extern __shared__ float lds[];
+    constexpr int E = PHASE ? 2 : 1;
+    const int line0 = blockIdx.x * tile_lines;
+    const int nl = min(tile_lines, c.num_lines - line0);
+    const int per_line = c.num_rf * E;
+    zero_lds(lds, nl * per_line);
+    __syncthreads();
+    const int begin = blockIdx.y * chunk;
+    const int end = min(s.n, begin + chunk);
+    for (int n = begin + threadIdx.x; n < end; n += blockDim.x) {
+        const float4 p = s.pts[n];
+        for (int k = 0; k < nl; k++)
+            project_point<ARC, PHASE, LUT>(c, c.lines[line0 + k], p.x, p.y, p.z, p.w, lds + k * per_line);
+    }
+    __syncthreads();
+    flush_bins(lds, bins + (size_t)line0 * per_line, nl * per_line, atomic != 0);
+}
+
+template <bool ARC, bool PHASE, bool LUT>
+__global__ __launch_bounds__(256) void project_spline_kernel(ProjCommon c, SplineSet s, float* bins,
+                                                             int tile_lines, int chunk, int atomic) {
+    extern __shared__ float lds[];
+    constexpr int E = PHASE ? 2 : 1;
+    const int line0 = blockIdx.x * tile_lines;
+    const int nl = min(tile_lines, c.num_lines - line0);
+    const int per_line = c.num_rf * E;
+    zero_lds(lds, nl * per_line);
+    __syncthreads();
+    const int begin = blockIdx.y * chunk;
+    const int end = min(s.n, begin + chunk);
+    const size_t N = (size_t)s.n;
+    for (int n = begin + threadIdx.x; n < end; n += blockDim.x) {
+        const float amp = s.amp[n];
+        for (int k = 0; k < nl; k++) {
+            const SplineLineBasis& B = s.basis[line0 + k];
+            // CpuAlgorithm.cpp:137-140: float sum from zero, in control-point order
+            float px = 0.0f, py = 0.0f, pz = 0.0f;
+            for (int i = 0; i < B.count; i++) {
+                const size_t idx = (size_t)(B.lo + i) * N + n;
+                px = px + s.cx[idx] * B.b[i];
+                py = py + s.cy[idx] * B.b[i];
+                pz = pz + s.cz[idx] * B.b[i];
+            }
+            project_point<ARC, PHASE, LUT>(c, c.lines[line0 + k], px, py, pz, amp, lds + k * per_line);
+        }
+    }
+    __syncthreads();
+    flush_bins(lds, bins + (size_t)line0 * per_line, nl * per_line, atomic != 0);
+}
+
+#define BCSIM_SELECT(kern, arc, phase, lut)                                                 \
+    ((arc) ? ((phase) ? ((lut) ? kern<true, true, true> : kern<true, true, false>)          \
+                      : ((lut) ? kern<true, false, true> : kern<true, false, false>))       \
+           : ((phase) ? ((lut) ? kern<false, true, true> : kern<false, true, false>)        \
+                      : ((lut) ? kern<false, false, true> : kern<false, false, false>)))
+
+static size_t proj_lds_bytes(const ProjCommon& c, bool complex_bins, int tile) {
+    return (size_t)tile * (size_t)c.num_rf * (complex_bins ? 8 : 4);
+}
+
+hipError_t launch_project_fixed(const ProjCommon& c, const FixedSet& s, void* bins, bool complex_bins,
+                                bool arc, bool phase, bool lut, const ProjLaunch& cfg,
+                                hipStream_t stream) {
+    if (s.n == 0) return hipSuccess;
+    const int tiles = (c.num_lines + cfg.tile_lines - 1) / cfg.tile_lines;
+    const int chunk = (s.n + cfg.slices - 1) / cfg.slices;
+    const size_t lds = proj_lds_bytes(c, complex_bins, cfg.tile_lines);
+    auto k = BCSIM_SELECT(project_fixed_kernel, arc, phase, lut);
+    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(tiles, cfg.slices), dim3(cfg.block), lds, stream, c, s,
+                       static_cast<float*>(bins), cfg.tile_lines, chunk, cfg.atomic);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_spline(const ProjCommon& c, const SplineSet& s, void* bins, bool complex_bins,
+                                 bool arc, bool phase, bool lut, const ProjLaunch& cfg,
+                                 hipStream_t stream) {
+    if (s.n == 0) return hipSuccess;
+    const int tiles = (c.num_lines + cfg.tile_lines - 1) / cfg.tile_lines;
+    const int chunk = (s.n + cfg.slices - 1) / cfg.slices;
+    const size_t lds = proj_lds_bytes(c, complex_bins, cfg.tile_lines);
+    auto k = BCSIM_SELECT(project_spline_kernel, arc, phase, lut);
+    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(tiles, cfg.slices), dim3(cfg.block), lds, stream, c, s,
+                       static_cast<float*>(bins), cfg.tile_lines, chunk, cfg.atomic);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// spline rendering at one time (all lines share a timestamp)
+
+__global__ __launch_bounds__(256) void render_spline_kernel(SplineSet s, SplineLineBasis B, float4* out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= s.n) return;
+    const size_t N = (size_t)s.n;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    for (int i = 0; i < B.count; i++) {
+        const size_t idx = (size_t)(B.lo + i) * N + n;
+        px = px + s.cx[idx] * B.b[i];
+        py = py + s.cy[idx] * B.b[i];
+        pz = pz + s.cz[idx] * B.b[i];
+    }
+    out[n] = make_float4(px, py, pz, s.amp[n]);
+}
+
+hipError_t launch_render_spline(const SplineSet& s, const SplineLineBasis& basis, float4* out,
+                                hipStream_t stream) {
+    if (s.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(render_spline_kernel, dim3((s.n + 255) / 256), dim3(256), 0, stream, s, basis, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// fused convolution + demodulation, one line per workgroup
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+#pragma clang fp contract(fast)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot90(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+struct Twiddle {
+    const float2* coarse;
+    const float2* fine;
+    int bits;
+    int mask;
+    // exp(-2 pi i idx / N) for 0 <= idx < N, conjugated for the inverse transform
+    template <bool INV>
+    __device__ __forceinline__ float2 get(int idx) const {
+        const float2 w = cmul(coarse[idx >> bits], fine[idx & mask]);
+        return INV ? make_float2(w.x, -w.y) : w;
+    }
+};
+
+// One Stockham autosort pass of radix R over N points held in LDS.  Ls is the
+// length of the sub-transforms already formed.  Every thread first reads all
+// its inputs, the workgroup synchronises, then every thread writes its outputs,
+// so one buffer suffices.
+template <int R, int NT, int NMAX, bool INV>
+__device__ __forceinline__ void stockham_pass(float2* buf, int N, int Ls, const Twiddle& tw) {
+#pragma clang fp contract(fast)
+    constexpr int IT = (NMAX / R + NT - 1) / NT;
+    const int m = N / R;
+    float2 x[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int j = threadIdx.x + it * NT;
+        if (j < m) {
+#pragma unroll
+            for (int q = 0; q < R; q++) x[it][q] = buf[j + q * m];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int j = threadIdx.x + it * NT;
+        if (j < m) {
+            const int k = j & (Ls - 1);
+            const int step = k * (N / (Ls * R));
+#pragma unroll
+            for (int q = 1; q < R; q++) x[it][q] = cmul(x[it][q], tw.template get<INV>(q * step));
+            float2 y[R];
+            if (R == 2) {
+                y[0] = cadd(x[it][0], x[it][1]);
+                y[1] = csub(x[it][0], x[it][1]);
+            } else {
+                const float2 a0 = cadd(x[it][0], x[it][2]);
+                const float2 a1 = csub(x[it][0], x[it][2]);
+                const float2 a2 = cadd(x[it][1], x[it][3]);
+                const float2 a3 = rot90<INV>(csub(x[it][1], x[it][3]));
+                y[0] = cadd(a0, a2);
+                y[2] = csub(a0, a2);
+                y[1] = cadd(a1, a3);
+                y[3] = csub(a1, a3);
+            }
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int s2 = 0; s2 < R; s2++) buf[base + s2 * Ls] = y[s2];
+        }
+    }
+    __syncthreads();
+}
+
+template <int NT, int NMAX, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf, int N, int log2n, const Twiddle& tw) {
+    int Ls = 1;
+    if (log2n & 1) {
+        stockham_pass<2, NT, NMAX, INV>(buf, N, Ls, tw);
+        Ls = 2;
+    }
+    while (Ls < N) {
+        stockham_pass<4, NT, NMAX, INV>(buf, N, Ls, tw);
+        Ls *= 4;
+    }
+}
+
+// Counter-based normal variates for the noise path: a 32-bit integer hash
+// (Wang's mix, then a multiply-xorshift round) feeds a Box-Muller transform.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x = (x ^ 61u) ^ (x >> 16);
+    x *= 9u;
+    x = x ^ (x >> 4);
+    x *= 0x27d4eb2du;
+    x = x ^ (x >> 15);
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    return x;
+}
+
+__device__ __forceinline__ float2 normal_pair(uint32_t seed, uint32_t counter, uint32_t line, uint32_t n) {
+    const uint32_t h0 = mix32(seed ^ mix32(counter ^ mix32(line * 0x9e3779b9u ^ mix32(n))));
+    const uint32_t h1 = mix32(h0 ^ 0x68bc21ebu);
+    const float u1 = ((h0 >> 8) + 1u) * (1.0f / 16777217.0f);  // (0, 1]
+    const float u2 = (h1 >> 8) * (1.0f / 16777216.0f);         // [0, 1)
+    const float rad = sqrtf(-2.0f * logf(u1));
+    float s, co;
+    sincospif(2.0f * u2, &s, &co);
+    return make_float2(rad * co, rad * s);
+}
+
+template <int NT, int NMAX, bool COMPLEX_BINS>
+__global__ __launch_bounds__(NT) void convolve_demod_kernel(ConvParams p) {
+    extern __shared__ float2 sbuf[];
+    const int N = p.fft_len;
+    float2* buf = sbuf;
+    float2* twc = sbuf + N;
+    const int n_coarse = N >> p.twiddle_bits;
+    const int n_fine = 1 << p.twiddle_bits;
+    float2* twf = twc + n_coarse;
+    const int line = blockIdx.x;
+    for (int i = threadIdx.x; i < n_coarse + n_fine; i += NT) twc[i] = p.twiddles[i];
+    // bins of this line, zero padded to N
+    if (COMPLEX_BINS) {
+        const float2* src = static_cast<const float2*>(p.bins) + (size_t)line * p.num_rf;
+        for (int n = threadIdx.x; n < N; n += NT) buf[n] = n < p.num_rf ? src[n] : make_float2(0.f, 0.f);
+    } else {
+        const float* src = static_cast<const float*>(p.bins) + (size_t)line * p.num_rf;
+        for (int n = threadIdx.x; n < N; n += NT) buf[n] = make_float2(n < p.num_rf ? src[n] : 0.f, 0.f);
+    }
+    if (p.noise_amplitude > 0.0f) {
+        for (int n = threadIdx.x; n < p.num_rf; n += NT) {
+            const float2 z = normal_pair(p.noise_seed, p.noise_counter, (uint32_t)line, (uint32_t)n);
+            buf[n] = make_float2(buf[n].x + p.noise_amplitude * z.x, buf[n].y + p.noise_amplitude * z.y);
+        }
+    }
+    __syncthreads();
+    Twiddle tw{twc, twf, p.twiddle_bits, n_fine - 1};
+    fft_lds<NT, NMAX, false>(buf, N, p.log2_len, tw);
+    // excitation spectrum with Hilbert mask and 1/N; zero above N/2 (the mask)
+    const int half = N / 2;
+    for (int k = threadIdx.x; k < N; k += NT) buf[k] = k <= half ? cmul(buf[k], p.H[k]) : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft_lds<NT, NMAX, true>(buf, N, p.log2_len, tw);
+    // delay compensation, demodulation, decimation
+    float2* out = p.iq + (size_t)line * p.n_out;
+    for (int j = threadIdx.x; j < p.n_out; j += NT) out[j] = cmul(buf[p.center + j * p.decimation], p.demod[j]);
+}
+
+int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
+
+hipError_t launch_convolve_demod(const ConvParams& p, hipStream_t stream) {
+    if (p.num_lines == 0) return hipSuccess;
+    const int nt = conv_block_threads(p.fft_len);
+    const size_t lds = ((size_t)p.fft_len + (p.fft_len >> p.twiddle_bits) + (1u << p.twiddle_bits)) * sizeof(float2);
+    const void* k;
+    if (nt == 1024) {
+        k = p.complex_bins ? (const void*)convolve_demod_kernel<1024, kMaxFftLen, true>
+                           : (const void*)convolve_demod_kernel<1024, kMaxFftLen, false>;
+    } else {
+        k = p.complex_bins ? (const void*)convolve_demod_kernel<256, 4096, true>
+                           : (const void*)convolve_demod_kernel<256, 4096, false>;
+    }
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    if (nt == 1024) {
+        if (p.complex_bins)
+            hipLaunchKernelGGL((convolve_demod_kernel<1024, kMaxFftLen, true>), dim3(p.num_lines), dim3(nt), lds, stream, p);
+        else
+            hipLaunchKernelGGL((convolve_demod_kernel<1024, kMaxFftLen, false>), dim3(p.num_lines), dim3(nt), lds, stream, p);
+    } else {
+        if (p.complex_bins)
+            hipLaunchKernelGGL((convolve_demod_kernel<256, 4096, true>), dim3(p.num_lines), dim3(nt), lds, stream, p);
+        else
+            hipLaunchKernelGGL((convolve_demod_kernel<256, 4096, false>), dim3(p.num_lines), dim3(nt), lds, stream, p);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bcsim_hip
