@@ -59,7 +59,10 @@ BCSIM_API bcsim_status bcsim_destroy(bcsim_sim* sim);
 /* IAlgorithm::set_parameter (LibBCSim.hpp:50; BaseAlgorithm.cpp:48-87,
  * GpuAlgorithm.cpp:68-107).  Keys: verbose, sound_speed, noise_amplitude,
  * use_arc_projection, radial_decimation, phase_delay, gpu_device,
- * cuda_streams, threads_per_block, store_kernel_details, noise_seed. */
+ * cuda_streams, threads_per_block, store_kernel_details; and two keys of this
+ * implementation: noise_seed (counter-based noise stream) and projection_mode
+ * ("tiled", the culled tile kernel, default; "direct", every line-scatterer
+ * pair, for checking). */
 BCSIM_API bcsim_status bcsim_set_parameter(bcsim_sim* sim, const char* key, const char* value);
 
 /* IAlgorithm::get_parameter (LibBCSim.hpp:53; GpuAlgorithm.cpp:761-773).

@@ -1,0 +1,129 @@
+// Device-side arithmetic of the scatterer projection, shared by the projection
+// kernels.  Every function here reproduces the rounding of the reference's CPU
+// code (CpuAlgorithm.cpp:50-183) where it selects a bin; translation units that
+// include it are compiled with -ffp-contract=off.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sim_device.h"
+
+namespace bcsim_hip {
+
+// Beam-local coordinates, float, left-to-right dot products (CpuAlgorithm.cpp:58-72).
+struct Local {
+    float r, l, e;
+};
+
+template <bool ARC>
+__device__ __forceinline__ Local local_coords(const LineGeom& L, float px, float py, float pz) {
+    const float dx = px - L.ox;
+    const float dy = py - L.oy;
+    const float dz = pz - L.oz;
+    Local o;
+    o.r = dx * L.rx + dy * L.ry + dz * L.rz;
+    o.l = dx * L.lx + dy * L.ly + dz * L.lz;
+    o.e = dx * L.ex + dy * L.ey + dz * L.ez;
+    if (ARC) o.r = copysignf(__builtin_sqrtf(dx * dx + dy * dy + dz * dz), o.r);
+    return o;
+}
+
+// GaussianBeamProfile::sampleProfile (BeamProfile.cpp:49-51).
+__device__ __forceinline__ float gauss_weight(const ProfileParams& P, float l, float e) {
+    return expf(-(l * l * P.inv_tsl + e * e * P.inv_tse));
+}
+
+// LUTBeamProfile::sampleProfile (BeamProfile.cpp:79-130): truncating indices, zero
+// whenever a neighbour falls outside the table, trilinear in r, then l, then e.
+__device__ __forceinline__ float lut_weight(const ProfileParams& P, float r, float l, float e) {
+    const float tr = (r - P.r_min) * P.inv_dr;
+    const float tl = (l - P.l_min) * P.inv_dl;
+    const float te = (e - P.e_min) * P.inv_de;
+    if (!(tr > -1.0f && tr < (float)(P.nr - 1))) return 0.0f;
+    if (!(tl > -1.0f && tl < (float)(P.nl - 1))) return 0.0f;
+    if (!(te > -1.0f && te < (float)(P.ne - 1))) return 0.0f;
+    const int r0 = (int)tr, r1 = (int)(tr + 1.0f);
+    const int l0 = (int)tl, l1 = (int)(tl + 1.0f);
+    const int e0 = (int)te, e1 = (int)(te + 1.0f);
+    if (r1 >= P.nr || l1 >= P.nl || e1 >= P.ne) return 0.0f;
+    const float fr = tr - (float)r0, fl = tl - (float)l0, fe = te - (float)e0;
+    const int sl = P.ne, sr = P.nl * P.ne;
+    const float* t = P.lut;
+    const float c000 = t[e0 + sl * l0 + sr * r0], c001 = t[e1 + sl * l0 + sr * r0];
+    const float c010 = t[e0 + sl * l1 + sr * r0], c011 = t[e1 + sl * l1 + sr * r0];
+    const float c100 = t[e0 + sl * l0 + sr * r1], c101 = t[e1 + sl * l0 + sr * r1];
+    const float c110 = t[e0 + sl * l1 + sr * r1], c111 = t[e1 + sl * l1 + sr * r1];
+    const float c00 = (1.0f - fr) * c000 + fr * c100;
+    const float c10 = (1.0f - fr) * c010 + fr * c110;
+    const float c01 = (1.0f - fr) * c001 + fr * c101;
+    const float c11 = (1.0f - fr) * c011 + fr * c111;
+    const float c0 = (1.0f - fl) * c00 + fl * c10;
+    const float c1 = (1.0f - fl) * c01 + fl * c11;
+    return (1.0f - fe) * c0 + fe * c1;
+}
+
+// Bin of radial distance r: floor(r*scale/denom + 0.5) evaluated in double as the
+// reference writes it (CpuAlgorithm.cpp:75 fixed, :161-162 spline).  The quotient is
+// first formed with the reciprocal; only when the result lies within 1e-9 of a
+// bin boundary (where the two could round differently) is the exact division
+// done.  Returns -1 for bins outside [0, num_rf) and for non-finite r.
+__device__ __forceinline__ int radial_bin(const ProjCommon& c, float r, double& true_index) {
+    const double q = (double)r * c.bin_scale;
+    true_index = q * c.bin_inv_denom;
+    double t = true_index + 0.5;
+    double fl = floor(t);
+    const double fr = t - fl;
+    if (fr < 1e-9 || fr > 1.0 - 1e-9) {
+        t = q / c.bin_denom + 0.5;
+        fl = floor(t);
+    }
+    if (!(t >= 0.0 && t < (double)c.num_rf)) return -1;
+    return (int)fl;
+}
+
+// Add one scatterer's contribution to its bin in LDS (CpuAlgorithm.cpp:77-95).
+template <bool PHASE, bool LUT>
+__device__ __forceinline__ void accumulate(const ProjCommon& c, float r, float l, float e, int bin,
+                                           double true_index, float amp, float* lbins) {
+    if (bin < 0) return;
+    const float w = (LUT ? lut_weight(c.prof, r, l, e) : gauss_weight(c.prof, l, e)) * amp;
+    if (w == 0.0f) return;  // adding +-0 leaves the bin unchanged
+    if (PHASE) {
+        // CpuAlgorithm.cpp:85-92: w * exp(i * 2 pi f_demod * (bin - true_index) / fs)
+        const float ph = (float)((double)bin - true_index) * c.phase_per_sample;
+        float s, co;
+        sincosf(ph, &s, &co);
+        atomicAdd(&lbins[2 * bin], w * co);
+        atomicAdd(&lbins[2 * bin + 1], w * s);
+    } else {
+        atomicAdd(&lbins[bin], w);
+    }
+}
+
+// One (line, scatterer) pair from scratch: local coordinates, bin, weight.
+template <bool ARC, bool PHASE, bool LUT>
+__device__ __forceinline__ void project_point(const ProjCommon& c, const LineGeom& L, float px, float py,
+                                              float pz, float amp, float* lbins) {
+    const Local q = local_coords<ARC>(L, px, py, pz);
+    double true_index;
+    const int bin = radial_bin(c, q.r, true_index);
+    accumulate<PHASE, LUT>(c, q.r, q.l, q.e, bin, true_index, amp, lbins);
+}
+
+__device__ __forceinline__ void zero_lds(float* lds, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = 0.0f;
+}
+
+// Write a tile of LDS bins to HBM: plain stores if this workgroup saw every
+// scatterer of the frame, else an atomic add of the non-zero bins.
+__device__ __forceinline__ void flush_bins(const float* lds, float* out, int n, bool atomic) {
+    if (atomic) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const float v = lds[i];
+            if (v != 0.0f) unsafeAtomicAdd(out + i, v);
+        }
+    } else {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = lds[i];
+    }
+}
+
+}  // namespace bcsim_hip

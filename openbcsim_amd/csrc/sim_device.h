@@ -110,6 +110,37 @@ struct ConvParams {
     uint32_t noise_counter;
 };
 
+// ---- culled tile projection (project_tile.hip) ----
+
+constexpr int kChunk = 64;      // scatterers per chunk: one per lane of a wave
+constexpr int kListCap = 512;   // chunk list entries staged in LDS per pass
+
+// Axis-aligned box around the (finite) positions a chunk can take: the points
+// of a fixed chunk, or every control point of a spline chunk (a B-spline stays
+// in the convex hull of its control points).  hx < 0 marks an empty chunk.
+struct alignas(16) ChunkBox {
+    float cx, cy, cz, pad0;
+    float hx, hy, hz, pad1;
+};
+
+// Support of the beam-profile weight in beam-local coordinates.  Outside it the
+// weight is exactly zero in float (LUT: outside the table, BeamProfile.cpp:97-105;
+// Gaussian: exponent beyond float's underflow), so skipping those pairs is exact.
+struct CullParams {
+    float r_lo, r_hi;         // radial: bin window [0, num_rf) and LUT r support
+    float l_lo, l_hi;         // LUT lateral support
+    float e_lo, e_hi;         // LUT elevational support
+    float g_a, g_b, g_max;    // Gaussian: a*l^2 + b*e^2 < g_max
+};
+
+struct TileLaunch {
+    int tile_lines;           // lines whose bins one workgroup keeps in LDS
+    int slices;               // chunk ranges per line tile (grid.y)
+    int chunks_per_slice;
+    int block;                // threads per workgroup (multiple of 64)
+    int atomic;               // flush with global atomics
+};
+
 // Launchers (sim_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_fill_zero(float* p, size_t n_floats, hipStream_t stream);
 hipError_t launch_project_fixed(const ProjCommon& c, const FixedSet& s, void* bins, bool complex_bins,
@@ -122,5 +153,13 @@ hipError_t launch_render_spline(const SplineSet& s, const SplineLineBasis& basis
                                 hipStream_t stream);
 hipError_t launch_convolve_demod(const ConvParams& p, hipStream_t stream);
 int conv_block_threads(int fft_len);
+
+// project_tile.hip: fixed (spline == nullptr) or spline dataset, chunked and
+// boxed by the host; `shared_origin` when every line starts at lines[0]'s origin.
+hipError_t launch_project_tile(const ProjCommon& c, const CullParams& cp, const FixedSet* fixed,
+                               const SplineSet* spline, const ChunkBox* boxes, int nchunks, void* bins,
+                               bool arc, bool phase, bool lut, bool shared_origin, const TileLaunch& cfg,
+                               hipStream_t stream);
+size_t tile_lds_bytes(int num_rf, bool complex_bins, int tile_lines);
 
 }  // namespace bcsim_hip

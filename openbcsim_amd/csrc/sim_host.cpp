@@ -76,19 +76,101 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Datasets are stored in Morton (z-curve) order of position, in chunks of
+// kChunk scatterers with one bounding box each, so the tile kernel can skip
+// whole chunks that lie outside a beam.  The order of the scatterers does not
+// change the frame beyond float summation order.
 struct FixedDataset {
-    DevBuf pts;  // float4[n]
+    DevBuf pts;    // float4[n] (x, y, z, amplitude), Morton order
+    DevBuf boxes;  // ChunkBox[nchunks]
     size_t n = 0;
+    int nchunks = 0;
 };
 
 struct SplineDataset {
     int degree = 0;
     int num_cp = 0;
     size_t n = 0;
+    int nchunks = 0;
     std::vector<float> knots;
-    DevBuf cx, cy, cz, amp;  // [num_cp][n] x3, [n]
+    DevBuf cx, cy, cz, amp;  // [num_cp][n] x3, [n], Morton order of the control-point centroid
+    DevBuf boxes;            // ChunkBox[nchunks] over all control points of the chunk
     DevBuf basis;            // SplineLineBasis[num_lines]
-    DevBuf rendered;         // float4[n], all-timestamps-equal path
+    DevBuf rendered;         // float4[n], all-timestamps-equal path (boxes: `boxes`, since a
+                             // rendered point lies in the box of its control points)
+};
+
+// 10 bits of each coordinate interleaved into a 30-bit z-curve key.
+uint32_t spread10(uint32_t x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+// Permutation that puts the points pos[i*stride .. +2] in z-curve order over
+// their bounding box; non-finite points go last.
+std::vector<uint32_t> morton_order(const float* pos, size_t n, size_t stride) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < n; i++) {
+        const float* p = pos + i * stride;
+        if (!finite3(p)) continue;
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> keyed(n);
+    for (size_t i = 0; i < n; i++) {
+        const float* p = pos + i * stride;
+        uint32_t key = 0xffffffffu;
+        if (finite3(p)) {
+            key = 0;
+            for (int a = 0; a < 3; a++) {
+                const double span = (double)hi[a] - (double)lo[a];
+                double q = span > 0 ? ((double)p[a] - (double)lo[a]) / span * 1024.0 : 0.0;
+                const uint32_t qi = (uint32_t)std::min(1023.0, std::max(0.0, q));
+                key |= spread10(qi) << a;
+            }
+        }
+        keyed[i] = {key, (uint32_t)i};
+    }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<uint32_t> perm(n);
+    for (size_t i = 0; i < n; i++) perm[i] = keyed[i].second;
+    return perm;
+}
+
+// Box of a set of points, padded by a relative 1e-6 so that rounding of the
+// centre and half extent never shrinks it.
+struct BoxAcc {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool any = false;
+    void add(const float* p) {
+        if (!finite3(p)) return;
+        any = true;
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    ChunkBox box() const {
+        ChunkBox b{0.f, 0.f, 0.f, 0.f, -1.f, -1.f, -1.f, 0.f};
+        if (!any) return b;
+        float c[3], h[3];
+        for (int a = 0; a < 3; a++) {
+            const double cd = 0.5 * ((double)lo[a] + (double)hi[a]);
+            c[a] = (float)cd;
+            const double hd = std::max((double)hi[a] - (double)c[a], (double)c[a] - (double)lo[a]);
+            h[a] = (float)(hd * (1.0 + 1e-6) + 1e-30);
+        }
+        b = ChunkBox{c[0], c[1], c[2], 0.f, h[0], h[1], h[2], 0.f};
+        return b;
+    }
 };
 
 // Vector3D<float>::dot evaluated left to right (vector3.hpp:86-88).
@@ -164,6 +246,7 @@ struct bcsim_sim {
     int num_streams = 2;
     int threads_per_block = 128;
     bool store_details = false;
+    bool direct_mode = false;  // projection_mode: "tiled" (culled tile kernel) or "direct"
 
     hipStream_t stream = nullptr;
     bcsim_log_fn log_fn = nullptr;
@@ -258,6 +341,10 @@ struct bcsim_sim {
             threads_per_block = n;  // launch shapes come from the tuner; kept for compatibility
         } else if (key == "store_kernel_details") {
             store_details = parse_bool(value);
+        } else if (key == "projection_mode") {
+            if (value == "tiled") direct_mode = false;
+            else if (value == "direct") direct_mode = true;
+            else fail(BCSIM_ERR_INVALID_ARGUMENT, "projection_mode must be 'tiled' or 'direct'");
         } else {
             fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal parameter name: '" + key + "'");
         }
@@ -285,6 +372,7 @@ struct bcsim_sim {
         if (key == "cuda_streams") return std::to_string(num_streams);
         if (key == "threads_per_block") return std::to_string(threads_per_block);
         if (key == "store_kernel_details") return store_details ? "on" : "off";
+        if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
         fail(BCSIM_ERR_INVALID_ARGUMENT, "Illegal key: " + key);
     }
 
@@ -346,8 +434,23 @@ struct bcsim_sim {
         use_device();
         auto ds = std::make_unique<FixedDataset>();
         ds->n = n;
+        ds->nchunks = (int)((n + kChunk - 1) / kChunk);
+        const std::vector<uint32_t> perm = morton_order(xyza, n, 4);
+        std::vector<float> sorted(n * 4);
+        for (size_t i = 0; i < n; i++) std::memcpy(&sorted[4 * i], xyza + 4 * (size_t)perm[i], 4 * sizeof(float));
+        std::vector<ChunkBox> boxes(ds->nchunks);
+        for (int c = 0; c < ds->nchunks; c++) {
+            BoxAcc acc;
+            for (size_t i = (size_t)c * kChunk; i < std::min(n, (size_t)(c + 1) * kChunk); i++) acc.add(&sorted[4 * i]);
+            boxes[c] = acc.box();
+        }
         ds->pts.ensure(n * sizeof(float4));
-        if (n) hip_check(hipMemcpy(ds->pts.p, xyza, n * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
+        ds->boxes.ensure(boxes.size() * sizeof(ChunkBox));
+        if (n) {
+            hip_check(hipMemcpy(ds->pts.p, sorted.data(), n * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(ds->boxes.p, boxes.data(), boxes.size() * sizeof(ChunkBox), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+        }
         fixed.push_back(std::move(ds));
         if (verbose) log(1, "Number of fixed scatterers: " + std::to_string(total_fixed()));
     }
@@ -369,25 +472,47 @@ struct bcsim_sim {
         ds->num_cp = num_cp;
         ds->n = n;
         ds->knots.assign(knots, knots + n_knots);
-        // [n][num_cp][3] -> three control-point-major planes [num_cp][n]
-        std::vector<float> px((size_t)num_cp * n), py((size_t)num_cp * n), pz((size_t)num_cp * n);
-        for (size_t s = 0; s < n; s++)
-            for (int i = 0; i < num_cp; i++) {
-                const float* c = cps + (s * num_cp + i) * 3;
-                px[(size_t)i * n + s] = c[0];
-                py[(size_t)i * n + s] = c[1];
-                pz[(size_t)i * n + s] = c[2];
+        ds->nchunks = (int)((n + kChunk - 1) / kChunk);
+        // order by the centroid of each spline's control points
+        std::vector<float> centroid(n * 3);
+        for (size_t s = 0; s < n; s++) {
+            double acc[3] = {0, 0, 0};
+            for (int i = 0; i < num_cp; i++)
+                for (int a = 0; a < 3; a++) acc[a] += cps[(s * num_cp + i) * 3 + a];
+            for (int a = 0; a < 3; a++) centroid[3 * s + a] = (float)(acc[a] / num_cp);
+        }
+        const std::vector<uint32_t> perm = morton_order(centroid.data(), n, 3);
+        // [n][num_cp][3] -> three control-point-major planes [num_cp][n], sorted
+        std::vector<float> px((size_t)num_cp * n), py((size_t)num_cp * n), pz((size_t)num_cp * n), am(n);
+        std::vector<ChunkBox> boxes(ds->nchunks);
+        for (int c = 0; c < ds->nchunks; c++) {
+            BoxAcc acc;
+            for (size_t s = (size_t)c * kChunk; s < std::min(n, (size_t)(c + 1) * kChunk); s++) {
+                const size_t src = perm[s];
+                am[s] = amps[src];
+                for (int i = 0; i < num_cp; i++) {
+                    const float* cp = cps + (src * num_cp + i) * 3;
+                    px[(size_t)i * n + s] = cp[0];
+                    py[(size_t)i * n + s] = cp[1];
+                    pz[(size_t)i * n + s] = cp[2];
+                    acc.add(cp);
+                }
             }
+            boxes[c] = acc.box();
+        }
         const size_t b = px.size() * sizeof(float);
         ds->cx.ensure(b);
         ds->cy.ensure(b);
         ds->cz.ensure(b);
         ds->amp.ensure(n * sizeof(float));
+        ds->boxes.ensure(boxes.size() * sizeof(ChunkBox));
         if (n) {
             hip_check(hipMemcpy(ds->cx.p, px.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
             hip_check(hipMemcpy(ds->cy.p, py.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
             hip_check(hipMemcpy(ds->cz.p, pz.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(hipMemcpy(ds->amp.p, amps, n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(ds->amp.p, am.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(ds->boxes.p, boxes.data(), boxes.size() * sizeof(ChunkBox), hipMemcpyHostToDevice),
+                      "hipMemcpy");
         }
         splines.push_back(std::move(ds));
         dirty_basis = true;
@@ -657,6 +782,69 @@ struct bcsim_sim {
         pending.clear();
     }
 
+    // Beam support for the chunk cull (see CullParams): the radial window that
+    // gives a bin in [0, num_rf), and the profile's non-zero region.
+    CullParams cull_params(const ProjCommon& c) const {
+        CullParams cp{};
+        const double per_sample = c.bin_denom / c.bin_scale;  // metres per bin
+        double r_lo = -0.5 * per_sample, r_hi = ((double)num_rf - 0.5) * per_sample;
+        cp.l_lo = cp.e_lo = -INFINITY;
+        cp.l_hi = cp.e_hi = INFINITY;
+        cp.g_a = cp.g_b = 0.0f;
+        cp.g_max = INFINITY;
+        if (profile == kProfileLut) {
+            const double dr = (double)((lut_r1 - lut_r0) / (float)(lut_nr - 1));
+            const double dl = (double)((lut_l1 - lut_l0) / (float)(lut_nl - 1));
+            const double de = (double)((lut_e1 - lut_e0) / (float)(lut_ne - 1));
+            r_lo = std::max(r_lo, (double)lut_r0 - std::fabs(dr));
+            r_hi = std::min(r_hi, (double)lut_r0 + (lut_nr - 1) * std::fabs(dr));
+            cp.l_lo = (float)((double)lut_l0 - std::fabs(dl));
+            cp.l_hi = (float)((double)lut_l0 + (lut_nl - 1) * std::fabs(dl));
+            cp.e_lo = (float)((double)lut_e0 - std::fabs(de));
+            cp.e_hi = (float)((double)lut_e0 + (lut_ne - 1) * std::fabs(de));
+            if (!(dr > 0 && dl > 0 && de > 0)) {  // degenerate table: no culling
+                r_lo = -INFINITY;
+                r_hi = INFINITY;
+                cp.l_lo = cp.e_lo = -INFINITY;
+                cp.l_hi = cp.e_hi = INFINITY;
+            }
+        } else {
+            // expf(-x) rounds to zero for x > 103.98 (below half the smallest denormal);
+            // 110 leaves a 6 % margin on the exponent
+            cp.g_a = c.prof.inv_tsl;
+            cp.g_b = c.prof.inv_tse;
+            cp.g_max = (std::isfinite(cp.g_a) && std::isfinite(cp.g_b)) ? 110.0f : INFINITY;
+        }
+        cp.r_lo = (float)r_lo;
+        cp.r_hi = (float)r_hi;
+        return cp;
+    }
+
+    bool shared_origin() const {
+        for (const auto& q : lines)
+            if (std::memcmp(&q.ox, &lines[0].ox, 3 * sizeof(float)) != 0) return false;
+        return true;
+    }
+
+    // Launch shape of the tile kernel: as many lines per workgroup as fit in
+    // LDS (at most 8), and enough chunk slices for about four workgroups per CU.
+    TileLaunch tile_launch(int nchunks) const {
+        TileLaunch cfg{};
+        const size_t per_line = (size_t)num_rf * (phase_delay ? 8 : 4);
+        const size_t avail = (size_t)kLdsBytes - (size_t)kListCap * 8 - 256;
+        int tile = (int)std::min<size_t>(8, avail / per_line);
+        tile = std::max(1, std::min(tile, (int)lines.size()));
+        const int tiles = ((int)lines.size() + tile - 1) / tile;
+        int slices = std::max(1, (1024 + tiles - 1) / tiles);
+        slices = std::min(slices, std::max(1, nchunks / 16));
+        cfg.tile_lines = tile;
+        cfg.chunks_per_slice = std::max(1, (nchunks + slices - 1) / slices);
+        cfg.slices = std::max(1, (nchunks + cfg.chunks_per_slice - 1) / cfg.chunks_per_slice);
+        cfg.block = 1024;
+        cfg.atomic = cfg.slices > 1 ? 1 : 0;
+        return cfg;
+    }
+
     void enqueue_frame(float2* d_out) {
         prepare();
         if (store_details) debug.clear();
@@ -664,36 +852,53 @@ struct bcsim_sim {
         const size_t L = lines.size();
         const size_t bin_floats = L * (size_t)num_rf * (cplx ? 2 : 1);
         float* bins = d_bins.as<float>();
+        const bool lut_profile = profile == kProfileLut;
+        const bool shared = shared_origin();
 
-        // Which datasets need an atomic flush: all but a single dataset projected
-        // by one slice per tile.  Zero the bins unless one launch writes them all.
-        const size_t ndatasets = fixed.size() + splines.size();
+        // One launch per non-empty dataset.  The bins are zeroed first unless a
+        // single launch writes every bin with plain stores.
+        size_t nonempty = 0;
+        for (auto& f : fixed) nonempty += f->n > 0;
+        for (auto& sp : splines) nonempty += sp->n > 0;
         bool need_zero = true;
-        if (ndatasets == 1) {
-            const size_t n = fixed.empty() ? splines[0]->n : fixed[0]->n;
-            need_zero = proj_launch(n).slices > 1 || n == 0;
+        if (nonempty == 1 && !direct_mode) {
+            int only = 0;
+            for (auto& f : fixed) if (f->n) only = f->nchunks;
+            for (auto& sp : splines) if (sp->n) only = sp->nchunks;
+            need_zero = tile_launch(only).slices > 1;
+        } else if (nonempty == 1 && direct_mode) {
+            size_t only = 0;
+            for (auto& f : fixed) if (f->n) only = f->n;
+            for (auto& sp : splines) if (sp->n) only = sp->n;
+            need_zero = proj_launch(only).slices > 1;
         }
         if (need_zero) {
             Timed t(this, "kernel_memset_ms");
             hip_check(launch_fill_zero(bins, bin_floats, stream), "fill kernel");
             t.stop();
         }
-        const bool lut_profile = profile == kProfileLut;
+        const bool force_atomic = nonempty > 1;
         for (auto& f : fixed) {
             if (f->n == 0) continue;
-            ProjLaunch cfg = proj_launch(f->n);
-            cfg.atomic = (cfg.slices > 1 || ndatasets > 1) ? 1 : 0;
             const ProjCommon c = proj_common(false);
+            const FixedSet fs{f->pts.as<float4>(), (int)f->n};
             Timed t(this, "fixed_projection_kernel_ms");
-            hip_check(launch_project_fixed(c, FixedSet{f->pts.as<float4>(), (int)f->n}, bins, cplx, arc,
-                                           phase_delay, lut_profile, cfg, stream),
-                      "fixed projection kernel");
+            if (direct_mode) {
+                ProjLaunch cfg = proj_launch(f->n);
+                cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
+                hip_check(launch_project_fixed(c, fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
+                          "fixed projection kernel");
+            } else {
+                TileLaunch cfg = tile_launch(f->nchunks);
+                cfg.atomic |= force_atomic ? 1 : 0;
+                hip_check(launch_project_tile(c, cull_params(c), &fs, nullptr, f->boxes.as<ChunkBox>(), f->nchunks,
+                                              bins, arc, phase_delay, lut_profile, shared, cfg, stream),
+                          "fixed projection kernel");
+            }
             t.stop();
         }
         for (auto& sp : splines) {
             if (sp->n == 0) continue;
-            ProjLaunch cfg = proj_launch(sp->n);
-            cfg.atomic = (cfg.slices > 1 || ndatasets > 1) ? 1 : 0;
             const ProjCommon c = proj_common(true);
             SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
                          sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
@@ -708,15 +913,37 @@ struct bcsim_sim {
                     hip_check(launch_render_spline(ss, B0, sp->rendered.as<float4>(), stream), "render kernel");
                     t.stop();
                 }
+                const FixedSet fs{sp->rendered.as<float4>(), (int)sp->n};
                 Timed t(this, "spline_projection_kernel_ms");
-                hip_check(launch_project_fixed(c, FixedSet{sp->rendered.as<float4>(), (int)sp->n}, bins, cplx,
-                                               arc, phase_delay, lut_profile, cfg, stream),
-                          "spline projection kernel");
+                if (direct_mode) {
+                    ProjLaunch cfg = proj_launch(sp->n);
+                    cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
+                    hip_check(launch_project_fixed(c, fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
+                              "spline projection kernel");
+                } else {
+                    TileLaunch cfg = tile_launch(sp->nchunks);
+                    cfg.atomic |= force_atomic ? 1 : 0;
+                    hip_check(launch_project_tile(c, cull_params(c), &fs, nullptr, sp->boxes.as<ChunkBox>(),
+                                                  sp->nchunks, bins, arc, phase_delay, lut_profile, shared, cfg,
+                                                  stream),
+                              "spline projection kernel");
+                }
                 t.stop();
             } else {
                 Timed t(this, "spline_projection_kernel_ms");
-                hip_check(launch_project_spline(c, ss, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
-                          "spline projection kernel");
+                if (direct_mode) {
+                    ProjLaunch cfg = proj_launch(sp->n);
+                    cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
+                    hip_check(launch_project_spline(c, ss, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
+                              "spline projection kernel");
+                } else {
+                    TileLaunch cfg = tile_launch(sp->nchunks);
+                    cfg.atomic |= force_atomic ? 1 : 0;
+                    hip_check(launch_project_tile(c, cull_params(c), nullptr, &ss, sp->boxes.as<ChunkBox>(),
+                                                  sp->nchunks, bins, arc, phase_delay, lut_profile, false, cfg,
+                                                  stream),
+                              "spline projection kernel");
+                }
                 t.stop();
             }
         }
