@@ -53,12 +53,15 @@ def projection_key(wl) -> str:
     return "fixed_projection_kernel_ms" if wl.fixed else "spline_projection_kernel_ms"
 
 
-def run_gpu(wl, steps: int, warmup: int):
+def run_gpu(wl, steps: int, warmup: int, tile=None):
     from openbcsim_amd.hip_rt import DeviceBuffer, hip
     from openbcsim_amd.pyrfsim import RfSimulator
 
     sim = RfSimulator("gpu")
     wl.apply(sim)
+    if tile:
+        for key, val in zip(("tile_windows", "tile_lines", "tile_block"), tile.split(",")):
+            sim.set_parameter(key, val)
     L, S = sim.output_shape()
     out = DeviceBuffer(L * S * 8)
     for _ in range(warmup):
@@ -138,7 +141,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="volume_lut")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile", default=None, help="projection launch shape 'windows,lines,block' (default: automatic)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a plain copy kernel of "
+                         "known size, to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration_mib > 0:
+        from openbcsim_amd.hip_rt import DeviceBuffer, hip
+        nbytes = args.copy_calibration_mib << 20
+        src, dst = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+        hip().hipMemcpy(C.c_void_p(dst.address), C.c_void_p(src.address), nbytes, 3)
+        hip().hipDeviceSynchronize()
+        src.close()
+        dst.close()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(lines shard across GPUs without communication; see the multi-GPU design notes)",
@@ -146,7 +161,7 @@ def main():
         sys.exit(2)
 
     wl = build_workload(args.workload)
-    L, S, ms, kern, host_ms = run_gpu(wl, args.steps, args.warmup)
+    L, S, ms, kern, host_ms = run_gpu(wl, args.steps, args.warmup, args.tile)
     N = wl.num_scatterers
     evals = L * N
     kind = "fixed" if wl.fixed else "spline"

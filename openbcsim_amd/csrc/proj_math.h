@@ -34,31 +34,67 @@ __device__ __forceinline__ float gauss_weight(const ProfileParams& P, float l, f
 
 // LUTBeamProfile::sampleProfile (BeamProfile.cpp:79-130): truncating indices, zero
 // whenever a neighbour falls outside the table, trilinear in r, then l, then e.
-__device__ __forceinline__ float lut_weight(const ProfileParams& P, float r, float l, float e) {
+// The table is read as quads: quad[(r*nl + l)*ne + e] holds the samples at
+// (l, e), (l, e+1), (l+1, e), (l+1, e+1) of radial slice r, so the eight corners
+// are two 16-byte loads.  Upper neighbour = lower + 1, except that a coordinate
+// in (-1, 0) truncates to 0 for both (the reference's int(t) and int(t + 1)).
+// The radial part depends on r only, so a caller projecting one point onto
+// several lines from a common origin computes it once (lut_radial).  The weight
+// feeds no discontinuity, so its interpolation uses fused lerps a + f*(b - a).
+struct LutRadial {
+    int off0, off1;  // quad offsets of radial slices r0 and r1
+    float fr;
+    bool ok;
+};
+
+__device__ __forceinline__ LutRadial lut_radial(const ProfileParams& P, float r) {
+    LutRadial R;
     const float tr = (r - P.r_min) * P.inv_dr;
+    R.ok = tr > -1.0f && tr < (float)(P.nr - 1);
+    const int r0 = R.ok ? (int)tr : 0;
+    const int slice = P.nl * P.ne;
+    R.off0 = r0 * slice;
+    R.off1 = (tr < 0.0f ? r0 : r0 + 1) * slice;
+    R.fr = tr - (float)r0;
+    return R;
+}
+
+__device__ __forceinline__ float lerp(float a, float b, float f) {
+#pragma clang fp contract(fast)
+    return a + f * (b - a);
+}
+
+__device__ __forceinline__ float lut_weight_le(const ProfileParams& P, const LutRadial& R, float l, float e) {
     const float tl = (l - P.l_min) * P.inv_dl;
     const float te = (e - P.e_min) * P.inv_de;
-    if (!(tr > -1.0f && tr < (float)(P.nr - 1))) return 0.0f;
-    if (!(tl > -1.0f && tl < (float)(P.nl - 1))) return 0.0f;
-    if (!(te > -1.0f && te < (float)(P.ne - 1))) return 0.0f;
-    const int r0 = (int)tr, r1 = (int)(tr + 1.0f);
-    const int l0 = (int)tl, l1 = (int)(tl + 1.0f);
-    const int e0 = (int)te, e1 = (int)(te + 1.0f);
-    if (r1 >= P.nr || l1 >= P.nl || e1 >= P.ne) return 0.0f;
-    const float fr = tr - (float)r0, fl = tl - (float)l0, fe = te - (float)e0;
-    const int sl = P.ne, sr = P.nl * P.ne;
-    const float* t = P.lut;
-    const float c000 = t[e0 + sl * l0 + sr * r0], c001 = t[e1 + sl * l0 + sr * r0];
-    const float c010 = t[e0 + sl * l1 + sr * r0], c011 = t[e1 + sl * l1 + sr * r0];
-    const float c100 = t[e0 + sl * l0 + sr * r1], c101 = t[e1 + sl * l0 + sr * r1];
-    const float c110 = t[e0 + sl * l1 + sr * r1], c111 = t[e1 + sl * l1 + sr * r1];
-    const float c00 = (1.0f - fr) * c000 + fr * c100;
-    const float c10 = (1.0f - fr) * c010 + fr * c110;
-    const float c01 = (1.0f - fr) * c001 + fr * c101;
-    const float c11 = (1.0f - fr) * c011 + fr * c111;
-    const float c0 = (1.0f - fl) * c00 + fl * c10;
-    const float c1 = (1.0f - fl) * c01 + fl * c11;
-    return (1.0f - fe) * c0 + fe * c1;
+    if (!(R.ok && tl > -1.0f && tl < (float)(P.nl - 1) && te > -1.0f && te < (float)(P.ne - 1))) return 0.0f;
+    const int l0 = (int)tl, e0 = (int)te;
+    const float fl = tl - (float)l0, fe = te - (float)e0;
+    const int row = l0 * P.ne + e0;
+    float4 a = P.quad[R.off0 + row];
+    float4 b = P.quad[R.off1 + row];
+    if (te < 0.0f || tl < 0.0f) {
+        if (te < 0.0f) {
+            a.y = a.x; a.w = a.z;
+            b.y = b.x; b.w = b.z;
+        }
+        if (tl < 0.0f) {
+            a.z = a.x; a.w = a.y;
+            b.z = b.x; b.w = b.y;
+        }
+    }
+    // radial, then lateral, then elevational (BeamProfile.cpp:117-129)
+    const float c00 = lerp(a.x, b.x, R.fr);
+    const float c10 = lerp(a.z, b.z, R.fr);
+    const float c01 = lerp(a.y, b.y, R.fr);
+    const float c11 = lerp(a.w, b.w, R.fr);
+    const float c0 = lerp(c00, c10, fl);
+    const float c1 = lerp(c01, c11, fl);
+    return lerp(c0, c1, fe);
+}
+
+__device__ __forceinline__ float lut_weight(const ProfileParams& P, float r, float l, float e) {
+    return lut_weight_le(P, lut_radial(P, r), l, e);
 }
 
 // Bin of radial distance r: floor(r*scale/denom + 0.5) evaluated in double as the

@@ -1,26 +1,35 @@
 // Culled tile projection for gfx950: the production scatterer-projection kernel.
 //
-// Work decomposition.  grid.x walks tiles of `tile_lines` scan lines whose bins
-// live in LDS for the whole kernel (up to ~156 KiB of the 160 KiB); grid.y
-// splits the scatterer chunks into slices.  The host stores every dataset in
-// Morton order of position and keeps one axis-aligned box per chunk of 64
-// scatterers (ChunkBox), so a chunk is spatially compact.
+// Work decomposition.  A workgroup owns a tile of `tile_lines` consecutive scan
+// lines (at most 32) and one radial window of `window_bins` bins of each of
+// them; the bins of that (tile, window) block live in LDS for the whole kernel.
+// Workgroup w = tile * windows + window, and the hardware deals workgroups to
+// the 8 XCDs round-robin, so with 8 windows each XCD handles one radial window
+// of every tile and touches only an eighth of the beam-profile table (measured:
+// L2 misses fall 20x on the 1M-scatterer volume).  That workload is bound by the
+// vector-memory path of the table gathers rather than by L2 misses, and one
+// window (whole lines in LDS) is the faster default; windows exist for lines
+// whose bins do not fit in LDS and for the tuner.  grid.y optionally splits the
+// chunks into slices.
 //
-// Per batch of up to kListCap chunks a workgroup
-//   (A) tests every chunk's box against the beam support of each line of its
-//       tile -- the region outside which the weight is exactly zero in float
-//       (LUT table extent, Gaussian underflow, bin window) -- and appends the
-//       surviving chunks with a per-line hit mask to a list in LDS;
-//   (B) hands the listed chunks to its waves, one chunk per wave, one scatterer
-//       per lane; each lane projects its scatterer onto every hit line of the
-//       tile and adds into the LDS bins with ds_add_f32.
-// The tile leaves LDS once: plain stores when the workgroup saw every chunk,
+// The host stores every dataset in Morton order of position and keeps one
+// axis-aligned box per chunk of 64 scatterers (ChunkBox).  Filling a chunk list
+// in LDS a step at a time and draining it when full, a workgroup
+//   (A) tests every chunk box -- first against the radial window, then against
+//       each line's beam support (outside which the weight is exactly zero in
+//       float: LUT table extent, Gaussian underflow) -- and appends survivors
+//       with a per-line hit mask to a list in LDS;
+//   (B) hands listed chunks to its waves, one chunk per wave and one scatterer
+//       per lane; each lane projects its scatterer onto every hit line and adds
+//       into the LDS bins with ds_add_f32.
+// The block leaves LDS once: plain stores when the workgroup saw every chunk,
 // otherwise atomic adds of the non-zero bins.
 //
-// Skipped pairs would have added w*a == 0 to a bin, so the sums are those of the
-// direct evaluation up to float summation order.  When every line starts at the
-// same origin (sector and pyramidal scans) the arc-projected distance |p - o|
-// and its bin are computed once per scatterer instead of once per line.
+// Skipped pairs would have added w*a == 0 to a bin, so the sums equal those of
+// the direct evaluation up to float summation order.  When every line starts at
+// the same origin (sector and pyramidal scans) the arc-projected distance
+// |p - o|, its bin and the radial part of the LUT lookup are computed once per
+// scatterer instead of once per line.
 //
 // Compiled with -ffp-contract=off (see proj_math.h).
 #include <hip/hip_runtime.h>
@@ -30,11 +39,12 @@
 
 namespace bcsim_hip {
 
-// Can any point of box B have a non-zero weight and a valid bin on line L?
-// Conservative: bounds are padded by a relative margin that covers the float
-// rounding of the per-point evaluation.
+// Can a point of box B have a non-zero weight and a bin inside [r_lo, r_hi) on
+// line L?  Conservative: bounds are padded by a relative margin that covers the
+// float rounding of the per-point evaluation.
 template <bool ARC, bool LUT>
-__device__ __forceinline__ bool chunk_hits(const CullParams& cp, const LineGeom& L, const ChunkBox& B) {
+__device__ __forceinline__ bool chunk_hits(const CullParams& cp, float r_lo, float r_hi, const LineGeom& L,
+                                           const ChunkBox& B) {
     if (!(B.hx >= 0.0f)) return false;  // empty chunk
     const float dx = B.cx - L.ox, dy = B.cy - L.oy, dz = B.cz - L.oz;
     const float hn = sqrtf(B.hx * B.hx + B.hy * B.hy + B.hz * B.hz);
@@ -43,88 +53,145 @@ __device__ __forceinline__ bool chunk_hits(const CullParams& cp, const LineGeom&
                              dn + hn) + 1e-30f;
     const float rc = dx * L.rx + dy * L.ry + dz * L.rz;
     const float hr = B.hx * fabsf(L.rx) + B.hy * fabsf(L.ry) + B.hz * fabsf(L.rz) + m;
-    const float lc = dx * L.lx + dy * L.ly + dz * L.lz;
-    const float hl = B.hx * fabsf(L.lx) + B.hy * fabsf(L.ly) + B.hz * fabsf(L.lz) + m;
-    const float ec = dx * L.ex + dy * L.ey + dz * L.ez;
-    const float he = B.hx * fabsf(L.ex) + B.hy * fabsf(L.ey) + B.hz * fabsf(L.ez) + m;
     bool hit_r;
     if (ARC) {
         // r = +-|p - o| with the sign of the projection onto the beam direction
         const float rmin = fmaxf(0.0f, dn - hn) - m, rmax = dn + hn + m;
-        const bool pos = (rc + hr > 0.0f) && (rmax > cp.r_lo) && (rmin < cp.r_hi);
-        const bool neg = (rc - hr < 0.0f) && (-rmin > cp.r_lo) && (-rmax < cp.r_hi);
+        const bool pos = (rc + hr > 0.0f) && (rmax > r_lo) && (rmin < r_hi);
+        const bool neg = (rc - hr < 0.0f) && (-rmin > r_lo) && (-rmax < r_hi);
         hit_r = pos || neg;
     } else {
-        hit_r = (rc + hr > cp.r_lo) && (rc - hr < cp.r_hi);
+        hit_r = (rc + hr > r_lo) && (rc - hr < r_hi);
     }
     if (!hit_r) return false;
+    const float lc = dx * L.lx + dy * L.ly + dz * L.lz;
+    const float hl = B.hx * fabsf(L.lx) + B.hy * fabsf(L.ly) + B.hz * fabsf(L.lz) + m;
+    const float ec = dx * L.ex + dy * L.ey + dz * L.ez;
+    const float he = B.hx * fabsf(L.ex) + B.hy * fabsf(L.ey) + B.hz * fabsf(L.ez) + m;
     if (LUT) return (lc + hl > cp.l_lo) && (lc - hl < cp.l_hi) && (ec + he > cp.e_lo) && (ec - he < cp.e_hi);
     const float dl = fmaxf(0.0f, fabsf(lc) - hl), de = fmaxf(0.0f, fabsf(ec) - he);
     return !(cp.g_a * dl * dl + cp.g_b * de * de >= cp.g_max);
 }
 
+// Radial-only test of a box against the window for lines from a common origin.
+__device__ __forceinline__ bool chunk_in_window(float r_lo, float r_hi, const LineGeom& L, const ChunkBox& B) {
+    if (!(B.hx >= 0.0f)) return false;
+    const float dx = B.cx - L.ox, dy = B.cy - L.oy, dz = B.cz - L.oz;
+    const float hn = sqrtf(B.hx * B.hx + B.hy * B.hy + B.hz * B.hz);
+    const float dn = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float m = 1e-6f * (fabsf(B.cx) + fabsf(B.cy) + fabsf(B.cz) + fabsf(L.ox) + fabsf(L.oy) + fabsf(L.oz) +
+                             dn + hn) + 1e-30f;
+    const float rmin = fmaxf(0.0f, dn - hn) - m, rmax = dn + hn + m;
+    return (rmax > r_lo && rmin < r_hi) || (-rmin > r_lo && -rmax < r_hi);
+}
+
+// Weight times amplitude added at window-relative bin lb (CpuAlgorithm.cpp:77-95).
+template <bool PHASE>
+__device__ __forceinline__ void add_bin(const ProjCommon& c, float* lbins, int lb, int bin, double true_index,
+                                        float w) {
+    if (w == 0.0f) return;  // adding +-0 leaves the bin unchanged
+    if (PHASE) {
+        const float ph = (float)((double)bin - true_index) * c.phase_per_sample;
+        float s, co;
+        sincosf(ph, &s, &co);
+        atomicAdd(&lbins[2 * lb], w * co);
+        atomicAdd(&lbins[2 * lb + 1], w * s);
+    } else {
+        atomicAdd(&lbins[lb], w);
+    }
+}
+
+template <bool PHASE, bool LUT>
+__device__ __forceinline__ void emit(const ProjCommon& c, float* lbins, int b0, int nb, float r, float l, float e,
+                                     int bin, double ti, float amp) {
+    const int lb = bin - b0;
+    if (bin < 0 || lb < 0 || lb >= nb) return;
+    const float w = (LUT ? lut_weight(c.prof, r, l, e) : gauss_weight(c.prof, l, e)) * amp;
+    add_bin<PHASE>(c, lbins, lb, bin, ti, w);
+}
+
 template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
 __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullParams cp, FixedSet fs, SplineSet ss,
                                                             const ChunkBox* __restrict__ boxes, int nchunks,
-                                                            float* __restrict__ out, int tile_lines,
-                                                            int chunks_per_slice, int atomic) {
+                                                            float* __restrict__ out, TileLaunch g) {
     extern __shared__ float lds[];
     __shared__ int list_n;
     constexpr int E = PHASE ? 2 : 1;
-    const int line0 = blockIdx.x * tile_lines;
-    const int nl = min(tile_lines, c.num_lines - line0);
-    const int per_line = c.num_rf * E;
+    const int window = blockIdx.x % g.windows;
+    const int tile = blockIdx.x / g.windows;
+    const int line0 = tile * g.tile_lines;
+    const int nl = min(g.tile_lines, c.num_lines - line0);
+    const int b0 = window * g.window_bins;
+    const int nb = min(g.window_bins, c.num_rf - b0);
+    const int stride = g.window_bins * E;  // LDS floats per line
     float* bins = lds;
-    int* list_chunk = reinterpret_cast<int*>(lds + (size_t)tile_lines * per_line);
-    unsigned* list_mask = reinterpret_cast<unsigned*>(list_chunk + kListCap);
-    zero_lds(bins, nl * per_line);
+    int* list_chunk = reinterpret_cast<int*>(lds + (size_t)g.tile_lines * stride);
+    unsigned* list_mask = reinterpret_cast<unsigned*>(list_chunk + g.list_cap);
+    zero_lds(bins, nl * stride);
+    if (threadIdx.x == 0) list_n = 0;
+
+    // radial extent of this window, intersected with the profile's radial support
+    const float r_lo = fmaxf(cp.r_lo, ((float)b0 - 0.5f) * cp.r_per_bin);
+    const float r_hi = fminf(cp.r_hi, ((float)(b0 + nb) - 0.5f) * cp.r_per_bin);
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
-    const int cbeg = blockIdx.y * chunks_per_slice;
-    const int cend = min(nchunks, cbeg + chunks_per_slice);
+    const int cbeg = blockIdx.y * g.chunks_per_slice;
+    const int cend = min(nchunks, cbeg + g.chunks_per_slice);
     const int N = SPLINE ? ss.n : fs.n;
 
-    for (int b0 = cbeg; b0 < cend; b0 += kListCap) {
-        if (threadIdx.x == 0) list_n = 0;
-        __syncthreads();
-        // (A) cull
-        const int b1 = min(cend, b0 + kListCap);
-        for (int ci = b0 + (int)threadIdx.x; ci < b1; ci += blockDim.x) {
+    // The list is filled `step` chunks at a time and drained when another step
+    // might not fit, so a workgroup synchronises about twice per step and the
+    // drains hand its waves long lists.
+    const int step = min((int)blockDim.x, g.list_cap);
+    __syncthreads();
+    for (int next = cbeg; next < cend;) {
+        // (A) cull the next `step` chunks
+        const int ci = next + (int)threadIdx.x;
+        if ((int)threadIdx.x < step && ci < cend) {
             const ChunkBox B = boxes[ci];
-            unsigned mask = 0;
-            for (int k = 0; k < nl; k++)
-                if (chunk_hits<ARC, LUT>(cp, c.lines[line0 + k], B)) mask |= 1u << k;
-            if (mask) {
-                const int slot = atomicAdd(&list_n, 1);
-                list_chunk[slot] = ci;
-                list_mask[slot] = mask;
+            if (!(SHARED && ARC) || chunk_in_window(r_lo, r_hi, c.lines[line0], B)) {
+                unsigned mask = 0;
+                for (int k = 0; k < nl; k++)
+                    if (chunk_hits<ARC, LUT>(cp, r_lo, r_hi, c.lines[line0 + k], B)) mask |= 1u << k;
+                if (mask) {
+                    const int slot = atomicAdd(&list_n, 1);
+                    list_chunk[slot] = ci;
+                    list_mask[slot] = mask;
+                }
             }
         }
+        next += step;
         __syncthreads();
-        // (B) project the surviving chunks, one wave per chunk
         const int n = list_n;
+        __syncthreads();
+        if (next < cend && n <= g.list_cap - step) continue;
+        // (B) project the listed chunks, one wave per chunk
         for (int j = wave; j < n; j += nwaves) {
             const int ci = list_chunk[j];
-            const unsigned mask = list_mask[j];
+            unsigned mask = list_mask[j];
             const int s = ci * kChunk + lane;
             if (s >= N) continue;
             if (SPLINE) {
                 const float amp = ss.amp[s];
-                const size_t stride = (size_t)N;
-                for (int k = 0; k < nl; k++) {
-                    if (!((mask >> k) & 1u)) continue;
+                const size_t pitch = (size_t)N;
+                while (mask) {
+                    const int k = __builtin_ctz(mask);
+                    mask &= mask - 1;
                     const SplineLineBasis& Bk = ss.basis[line0 + k];
                     // CpuAlgorithm.cpp:137-140: float sum from zero, in control-point order
                     float px = 0.0f, py = 0.0f, pz = 0.0f;
                     for (int i = 0; i < Bk.count; i++) {
-                        const size_t idx = (size_t)(Bk.lo + i) * stride + s;
+                        const size_t idx = (size_t)(Bk.lo + i) * pitch + s;
                         px = px + ss.cx[idx] * Bk.b[i];
                         py = py + ss.cy[idx] * Bk.b[i];
                         pz = pz + ss.cz[idx] * Bk.b[i];
                     }
-                    project_point<ARC, PHASE, LUT>(c, c.lines[line0 + k], px, py, pz, amp, bins + k * per_line);
+                    const Local q = local_coords<ARC>(c.lines[line0 + k], px, py, pz);
+                    double ti;
+                    const int bin = radial_bin(c, q.r, ti);
+                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, amp);
                 }
             } else if (SHARED && ARC) {
                 const float4 p = fs.pts[s];
@@ -133,36 +200,62 @@ __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullPa
                 const float absd = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
                 double ti_pos;
                 const int bin_pos = radial_bin(c, absd, ti_pos);
-                for (int k = 0; k < nl; k++) {
-                    if (!((mask >> k) & 1u)) continue;
+                const int lb_pos = bin_pos - b0;
+                const bool in_pos = bin_pos >= 0 && lb_pos >= 0 && lb_pos < nb;
+                LutRadial R{};
+                if (LUT) R = lut_radial(c.prof, absd);
+                while (mask) {
+                    const int k = __builtin_ctz(mask);
+                    mask &= mask - 1;
                     const LineGeom& L = c.lines[line0 + k];
                     const float rp = dx * L.rx + dy * L.ry + dz * L.rz;
-                    const float l = dx * L.lx + dy * L.ly + dz * L.lz;
-                    const float e = dx * L.ex + dy * L.ey + dz * L.ez;
                     if (!signbit(rp)) {
-                        accumulate<PHASE, LUT>(c, absd, l, e, bin_pos, ti_pos, p.w, bins + k * per_line);
+                        if (!in_pos) continue;
+                        const float l = dx * L.lx + dy * L.ly + dz * L.lz;
+                        const float e = dx * L.ex + dy * L.ey + dz * L.ez;
+                        const float w = (LUT ? lut_weight_le(c.prof, R, l, e) : gauss_weight(c.prof, l, e)) * p.w;
+                        add_bin<PHASE>(c, bins + k * stride, lb_pos, bin_pos, ti_pos, w);
                     } else {
+                        const float l = dx * L.lx + dy * L.ly + dz * L.lz;
+                        const float e = dx * L.ex + dy * L.ey + dz * L.ez;
                         double ti;
                         const int bin = radial_bin(c, -absd, ti);
-                        accumulate<PHASE, LUT>(c, -absd, l, e, bin, ti, p.w, bins + k * per_line);
+                        emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w);
                     }
                 }
             } else {
                 const float4 p = fs.pts[s];
-                for (int k = 0; k < nl; k++) {
-                    if (!((mask >> k) & 1u)) continue;
-                    project_point<ARC, PHASE, LUT>(c, c.lines[line0 + k], p.x, p.y, p.z, p.w, bins + k * per_line);
+                while (mask) {
+                    const int k = __builtin_ctz(mask);
+                    mask &= mask - 1;
+                    const Local q = local_coords<ARC>(c.lines[line0 + k], p.x, p.y, p.z);
+                    double ti;
+                    const int bin = radial_bin(c, q.r, ti);
+                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, p.w);
                 }
             }
         }
         __syncthreads();
+        if (threadIdx.x == 0) list_n = 0;
+        __syncthreads();
     }
     __syncthreads();
-    flush_bins(bins, out + (size_t)line0 * per_line, nl * per_line, atomic != 0);
+    // flush: line k's window [b0, b0 + nb) of the global bins [line][num_rf]
+    const int row = nb * E;
+    for (int i = threadIdx.x; i < nl * row; i += blockDim.x) {
+        const int k = i / row, o = i - k * row;
+        const float v = bins[k * stride + o];
+        float* dst = out + ((size_t)(line0 + k) * c.num_rf + b0) * E + o;
+        if (g.atomic) {
+            if (v != 0.0f) unsafeAtomicAdd(dst, v);
+        } else {
+            *dst = v;
+        }
+    }
 }
 
-size_t tile_lds_bytes(int num_rf, bool complex_bins, int tile_lines) {
-    return (size_t)tile_lines * (size_t)num_rf * (complex_bins ? 8 : 4) + (size_t)kListCap * 8;
+size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap) {
+    return (size_t)tile_lines * (size_t)window_bins * (complex_bins ? 8 : 4) + (size_t)list_cap * 8;
 }
 
 template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
@@ -170,12 +263,12 @@ static hipError_t launch_tile_t(const ProjCommon& c, const CullParams& cp, const
                                 const ChunkBox* boxes, int nchunks, float* bins, const TileLaunch& cfg,
                                 hipStream_t stream) {
     const int tiles = (c.num_lines + cfg.tile_lines - 1) / cfg.tile_lines;
-    const size_t lds = tile_lds_bytes(c.num_rf, PHASE, cfg.tile_lines);
+    const size_t lds = tile_lds_bytes(cfg.window_bins, PHASE, cfg.tile_lines, cfg.list_cap);
     auto k = project_tile_kernel<ARC, PHASE, LUT, SPLINE, SHARED>;
     hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(tiles, cfg.slices), dim3(cfg.block), lds, stream, c, cp, fs, ss, boxes, nchunks,
-                       bins, cfg.tile_lines, cfg.chunks_per_slice, cfg.atomic);
+    hipLaunchKernelGGL(k, dim3(tiles * cfg.windows, cfg.slices), dim3(cfg.block), lds, stream, c, cp, fs, ss, boxes,
+                       nchunks, bins, cfg);
     return hipGetLastError();
 }
 

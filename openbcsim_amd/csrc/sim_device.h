@@ -46,8 +46,9 @@ enum ProfileKind { kProfileGaussian = 0, kProfileLut = 1 };
 struct ProfileParams {
     // Gaussian: exp(-(l*l*inv_tsl + e*e*inv_tse)), inv_ts* = 1/(2 sigma^2)
     float inv_tsl, inv_tse;
-    // LUT (BeamProfile.cpp:79-130): t = (x - x_min) * inv_dx, trilinear, zero outside
-    const float* lut;
+    // LUT (BeamProfile.cpp:79-130): t = (x - x_min) * inv_dx, trilinear, zero outside;
+    // read as quads (see lut_weight): quad[(r*nl + l)*ne + e]
+    const float4* quad;
     int nr, nl, ne;
     float r_min, l_min, e_min;
     float inv_dr, inv_dl, inv_de;
@@ -113,7 +114,7 @@ struct ConvParams {
 // ---- culled tile projection (project_tile.hip) ----
 
 constexpr int kChunk = 64;      // scatterers per chunk: one per lane of a wave
-constexpr int kListCap = 512;   // chunk list entries staged in LDS per pass
+constexpr int kMaxListCap = 2048;  // chunk list entries held in LDS between drains
 
 // Axis-aligned box around the (finite) positions a chunk can take: the points
 // of a fixed chunk, or every control point of a spline chunk (a B-spline stays
@@ -127,6 +128,7 @@ struct alignas(16) ChunkBox {
 // weight is exactly zero in float (LUT: outside the table, BeamProfile.cpp:97-105;
 // Gaussian: exponent beyond float's underflow), so skipping those pairs is exact.
 struct CullParams {
+    float r_per_bin;          // metres of radial distance per bin (denom / scale)
     float r_lo, r_hi;         // radial: bin window [0, num_rf) and LUT r support
     float l_lo, l_hi;         // LUT lateral support
     float e_lo, e_hi;         // LUT elevational support
@@ -134,11 +136,14 @@ struct CullParams {
 };
 
 struct TileLaunch {
-    int tile_lines;           // lines whose bins one workgroup keeps in LDS
-    int slices;               // chunk ranges per line tile (grid.y)
+    int tile_lines;           // lines whose bins one workgroup keeps in LDS (<= 32)
+    int windows;              // radial windows per line; workgroup = tile * windows + window
+    int window_bins;          // bins per window (the last window may be shorter)
+    int slices;               // chunk ranges per (tile, window) (grid.y)
     int chunks_per_slice;
     int block;                // threads per workgroup (multiple of 64)
     int atomic;               // flush with global atomics
+    int list_cap;             // chunk list entries in LDS (multiple of 64)
 };
 
 // Launchers (sim_kernels.hip).  All enqueue on `stream` and return the launch status.
@@ -160,6 +165,6 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams& cp, const 
                                const SplineSet* spline, const ChunkBox* boxes, int nchunks, void* bins,
                                bool arc, bool phase, bool lut, bool shared_origin, const TileLaunch& cfg,
                                hipStream_t stream);
-size_t tile_lds_bytes(int num_rf, bool complex_bins, int tile_lines);
+size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap);
 
 }  // namespace bcsim_hip

@@ -247,6 +247,10 @@ struct bcsim_sim {
     int threads_per_block = 128;
     bool store_details = false;
     bool direct_mode = false;  // projection_mode: "tiled" (culled tile kernel) or "direct"
+    // launch shape of the tile kernel; 0 = chosen from the frame shape
+    struct {
+        int windows = 0, lines = 0, block = 0, slices = 0;
+    } tune;
 
     hipStream_t stream = nullptr;
     bcsim_log_fn log_fn = nullptr;
@@ -272,7 +276,7 @@ struct bcsim_sim {
     // profile
     int profile = -1;
     float sigma_l = 0.f, sigma_e = 0.f;
-    DevBuf lut;
+    DevBuf lut;  // float4 quads [n_rad][n_lat][n_ele]
     int lut_nr = 0, lut_nl = 0, lut_ne = 0;
     float lut_r0 = 0, lut_r1 = 0, lut_l0 = 0, lut_l1 = 0, lut_e0 = 0, lut_e1 = 0;
 
@@ -345,6 +349,16 @@ struct bcsim_sim {
             if (value == "tiled") direct_mode = false;
             else if (value == "direct") direct_mode = true;
             else fail(BCSIM_ERR_INVALID_ARGUMENT, "projection_mode must be 'tiled' or 'direct'");
+        } else if (key == "tile_windows" || key == "tile_lines" || key == "tile_block" || key == "tile_slices") {
+            const int v = parse_int(value);
+            if (v < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, key + " must be >= 0 (0 = automatic)");
+            if (key == "tile_lines" && v > 32) fail(BCSIM_ERR_INVALID_ARGUMENT, "tile_lines must be <= 32");
+            if (key == "tile_block" && (v % 64 != 0 || v > 1024))
+                fail(BCSIM_ERR_INVALID_ARGUMENT, "tile_block must be a multiple of 64, at most 1024");
+            if (key == "tile_windows") tune.windows = v;
+            if (key == "tile_lines") tune.lines = v;
+            if (key == "tile_block") tune.block = v;
+            if (key == "tile_slices") tune.slices = v;
         } else {
             fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal parameter name: '" + key + "'");
         }
@@ -373,6 +387,10 @@ struct bcsim_sim {
         if (key == "threads_per_block") return std::to_string(threads_per_block);
         if (key == "store_kernel_details") return store_details ? "on" : "off";
         if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
+        if (key == "tile_windows") return std::to_string(tune.windows);
+        if (key == "tile_lines") return std::to_string(tune.lines);
+        if (key == "tile_block") return std::to_string(tune.block);
+        if (key == "tile_slices") return std::to_string(tune.slices);
         fail(BCSIM_ERR_INVALID_ARGUMENT, "Illegal key: " + key);
     }
 
@@ -533,9 +551,19 @@ struct bcsim_sim {
         if (ne <= 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "Too few elevational samples");
         if (!samples) fail(BCSIM_ERR_INVALID_ARGUMENT, "null LUT samples");
         use_device();
-        const size_t bytes = (size_t)nr * nl * ne * sizeof(float);
-        lut.ensure(bytes);
-        hip_check(hipMemcpy(lut.p, samples, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        // quads of the (l, e) neighbourhood, see lut_weight in proj_math.h
+        const size_t cells = (size_t)nr * nl * ne;
+        std::vector<float4> quad(cells);
+        auto at = [&](int r, int l, int e) -> float {
+            return (l < nl && e < ne) ? samples[((size_t)r * nl + l) * ne + e] : 0.0f;
+        };
+        for (int r = 0; r < nr; r++)
+            for (int l = 0; l < nl; l++)
+                for (int e = 0; e < ne; e++)
+                    quad[((size_t)r * nl + l) * ne + e] =
+                        make_float4(at(r, l, e), at(r, l, e + 1), at(r, l + 1, e), at(r, l + 1, e + 1));
+        lut.ensure(cells * sizeof(float4));
+        hip_check(hipMemcpy(lut.p, quad.data(), cells * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
         lut_nr = nr;
         lut_nl = nl;
         lut_ne = ne;
@@ -711,7 +739,7 @@ struct bcsim_sim {
             P.inv_tsl = (float)(1.0 / (2.0 * (double)sigma_l * (double)sigma_l));
             P.inv_tse = (float)(1.0 / (2.0 * (double)sigma_e * (double)sigma_e));
         } else {
-            P.lut = lut.as<float>();
+            P.quad = lut.as<float4>();
             P.nr = lut_nr;
             P.nl = lut_nl;
             P.ne = lut_ne;
@@ -815,6 +843,7 @@ struct bcsim_sim {
             cp.g_b = c.prof.inv_tse;
             cp.g_max = (std::isfinite(cp.g_a) && std::isfinite(cp.g_b)) ? 110.0f : INFINITY;
         }
+        cp.r_per_bin = (float)per_sample;
         cp.r_lo = (float)r_lo;
         cp.r_hi = (float)r_hi;
         return cp;
@@ -830,18 +859,40 @@ struct bcsim_sim {
     // LDS (at most 8), and enough chunk slices for about four workgroups per CU.
     TileLaunch tile_launch(int nchunks) const {
         TileLaunch cfg{};
-        const size_t per_line = (size_t)num_rf * (phase_delay ? 8 : 4);
-        const size_t avail = (size_t)kLdsBytes - (size_t)kListCap * 8 - 256;
-        int tile = (int)std::min<size_t>(8, avail / per_line);
-        tile = std::max(1, std::min(tile, (int)lines.size()));
-        const int tiles = ((int)lines.size() + tile - 1) / tile;
-        int slices = std::max(1, (1024 + tiles - 1) / tiles);
-        slices = std::min(slices, std::max(1, nchunks / 16));
-        cfg.tile_lines = tile;
+        const int L = (int)lines.size();
+        const size_t bin_bytes = phase_delay ? 8 : 4;
+        const size_t avail = (size_t)kLdsBytes - 64 * 8 - 256;  // keep room for a 64-entry list
+        // radial windows: one by default (measured fastest on the workloads of the tests
+        // and bench); more if one line's bins do not fit in LDS
+        int K = tune.windows > 0 ? tune.windows : 1;
+        K = std::max(1, std::min(K, num_rf));
+        while ((size_t)((num_rf + K - 1) / K) * bin_bytes > avail) K++;
+        const int W = (num_rf + K - 1) / K;
+        K = (num_rf + W - 1) / W;
+        const int tmax = (int)std::max<size_t>(1, std::min<size_t>(32, avail / ((size_t)W * bin_bytes)));
+        int T;
+        if (tune.lines > 0) {
+            T = std::min(tune.lines, tmax);
+        } else {
+            // as many lines as fit, but keep at least ~1024 workgroups
+            T = std::min(tmax, 16);
+            const int want = std::max(1, (L * K) / 1024);
+            T = std::max(1, std::min(T, want));
+        }
+        T = std::max(1, std::min(T, L));
+        const int blocks = ((L + T - 1) / T) * K;
+        int slices = tune.slices > 0 ? tune.slices : std::max(1, (512 + blocks - 1) / blocks);
+        slices = std::max(1, std::min(slices, std::max(1, nchunks / 16)));
+        cfg.tile_lines = T;
+        cfg.windows = K;
+        cfg.window_bins = W;
         cfg.chunks_per_slice = std::max(1, (nchunks + slices - 1) / slices);
         cfg.slices = std::max(1, (nchunks + cfg.chunks_per_slice - 1) / cfg.chunks_per_slice);
-        cfg.block = 1024;
+        cfg.block = tune.block > 0 ? tune.block : 1024;
         cfg.atomic = cfg.slices > 1 ? 1 : 0;
+        // the rest of the LDS holds the chunk list
+        const size_t left = (size_t)kLdsBytes - 256 - (size_t)T * W * bin_bytes;
+        cfg.list_cap = (int)std::min<size_t>(kMaxListCap, (left / 8) / 64 * 64);
         return cfg;
     }
 
