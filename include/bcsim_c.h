@@ -162,6 +162,36 @@ BCSIM_API bcsim_status bcsim_set_logger(bcsim_sim* sim, bcsim_log_fn fn, void* u
 BCSIM_API bcsim_status bcsim_debug_get_projection(bcsim_sim* sim, float* out, size_t capacity,
                                                   int* num_lines, int* num_rf);
 
+/* ------------------------------------------------------------------------------------------
+ * Batched frames.  No reference counterpart: the reference simulates one frame per
+ * simulate_lines call with per-line launches (GpuAlgorithm.cpp:189-360).  A batch
+ * holds configured simulator handles (jobs); bcsim_batch_simulate* runs one frame of
+ * every job with one persistent projection launch and one persistent convolution
+ * launch, captured into a HIP graph and replayed while no job changes.  Each job's
+ * IQ is written at its offset of one complex64 output (bcsim_batch_get_layout).
+ * A handle in a batch must not run frames of its own concurrently with the batch.
+ */
+typedef struct bcsim_batch bcsim_batch;
+
+BCSIM_API bcsim_status bcsim_batch_create(bcsim_batch** out);
+BCSIM_API bcsim_status bcsim_batch_destroy(bcsim_batch* batch);
+/* Keys: use_graph (on/off, default on), store_kernel_details (on/off: records
+ * batch_frame_ms with HIP events, waiting for each frame). */
+BCSIM_API bcsim_status bcsim_batch_set_parameter(bcsim_batch* batch, const char* key, const char* value);
+BCSIM_API bcsim_status bcsim_batch_add(bcsim_batch* batch, bcsim_sim* sim, int* job_index);
+BCSIM_API bcsim_status bcsim_batch_clear(bcsim_batch* batch);
+/* total complex elements of the output; per job (up to capacity jobs) its offset,
+ * lines and samples per line. */
+BCSIM_API bcsim_status bcsim_batch_get_layout(bcsim_batch* batch, size_t* total, int* num_jobs,
+                                              size_t* offsets, int* lines, int* samples, int capacity);
+/* Enqueue one frame of every job, IQ to device memory d_out; returns without waiting. */
+BCSIM_API bcsim_status bcsim_batch_simulate_device(bcsim_batch* batch, void* d_out, size_t capacity);
+/* One frame of every job into host memory; returns when the data is there. */
+BCSIM_API bcsim_status bcsim_batch_simulate(bcsim_batch* batch, float* out, size_t capacity);
+BCSIM_API bcsim_status bcsim_batch_synchronize(bcsim_batch* batch);
+BCSIM_API bcsim_status bcsim_batch_get_debug_data(bcsim_batch* batch, const char* key, double* out,
+                                                  size_t capacity, size_t* count);
+
 #ifdef __cplusplus
 }
 #endif

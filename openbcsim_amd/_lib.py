@@ -44,6 +44,16 @@ SIGNATURES = {
     "bcsim_get_total_num_scatterers": [_h, _sp],
     "bcsim_set_logger": [_h, LOG_FN, C.c_void_p],
     "bcsim_debug_get_projection": [_h, _fp, C.c_size_t, _ip, _ip],
+    "bcsim_batch_create": [C.POINTER(C.c_void_p)],
+    "bcsim_batch_destroy": [_h],
+    "bcsim_batch_set_parameter": [_h, C.c_char_p, C.c_char_p],
+    "bcsim_batch_add": [_h, _h, _ip],
+    "bcsim_batch_clear": [_h],
+    "bcsim_batch_get_layout": [_h, _sp, _ip, _sp, _ip, _ip, C.c_int],
+    "bcsim_batch_simulate_device": [_h, C.c_void_p, C.c_size_t],
+    "bcsim_batch_simulate": [_h, _fp, C.c_size_t],
+    "bcsim_batch_synchronize": [_h],
+    "bcsim_batch_get_debug_data": [_h, C.c_char_p, _dp, C.c_size_t, _sp],
 }
 
 _lib = None

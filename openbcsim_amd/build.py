@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={A
           "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include"), "-fvisibility=hidden"]
 
-SOURCES = ["sim_kernels.hip", "project_tile.hip", "sim_host.cpp"]
+SOURCES = ["sim_kernels.hip", "project_tile.hip", "sim_host.cpp", "batch_host.cpp"]
 
 
 def _run(cmd):

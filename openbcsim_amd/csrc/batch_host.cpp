@@ -1,0 +1,390 @@
+// Batched frames: many independent simulators (jobs) of uneven size simulated by
+// one persistent projection launch and one persistent convolution launch.
+//
+// The reference has no batched path: every simulate_lines call is its own
+// sequence of per-line launches (GpuAlgorithm.cpp:189-360).  Small jobs cannot
+// fill 256 CUs on their own, so running many of them as separate calls leaves
+// the GPU mostly idle; here each job keeps its own resident phantom, tables and
+// bins, and the batch
+//   1. zeroes the bins of jobs that accumulate and resets the work counters
+//      (one fill kernel, no memset nodes, so the sequence can be captured),
+//   2. renders all-timestamps-equal spline sets,
+//   3. runs one persistent projection kernel over (job, tile-block) items,
+//      heaviest items first,
+//   4. runs one persistent convolution kernel over (job, line) items, longest
+//      transforms first, writing each job's IQ at its offset of the output.
+// The sequence is captured once into a HIP graph and replayed while no job's
+// configuration and the output pointer stay the same.
+#include "sim_state.h"
+
+#include <numeric>
+
+namespace {
+
+struct JobPlan {
+    bcsim_sim* sim;
+    uint64_t version;
+    size_t out_offset;  // complex elements
+    int lines, samples;
+};
+
+}  // namespace
+
+struct bcsim_batch {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    bool use_graph = true;
+    bool store_details = false;
+    std::vector<bcsim_sim*> sims;
+
+    // plan of the last build
+    bool planned = false;
+    std::vector<JobPlan> plan;
+    size_t total_out = 0;
+    DevBuf d_proj_jobs, d_proj_items, d_conv_jobs, d_conv_items, d_fill, d_counters;
+    int n_proj_items = 0, n_conv_items = 0, n_fill = 0;
+    size_t proj_lds = 0;
+    struct Render {
+        SplineSet set;
+        SplineLineBasis basis;
+        float4* out;
+    };
+    std::vector<Render> renders;
+    int workgroups = 256;
+
+    // graph of the last capture
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* graph_out = nullptr;
+
+    std::map<std::string, std::vector<double>> debug;
+
+    ~bcsim_batch() {
+        drop_graph();
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_out = nullptr;
+    }
+
+    void invalidate() {
+        planned = false;
+        drop_graph();
+    }
+
+    bool stale() const {
+        if (!planned || plan.size() != sims.size()) return true;
+        for (size_t i = 0; i < sims.size(); i++)
+            if (plan[i].sim != sims[i] || plan[i].version != sims[i]->version) return true;
+        return false;
+    }
+
+    void add(bcsim_sim* s) {
+        need(s);
+        if (device < 0) device = s->device;
+        if (s->device != device) fail(BCSIM_ERR_INVALID_ARGUMENT, "all simulators of a batch must use one device");
+        sims.push_back(s);
+        invalidate();
+    }
+
+    // Build the device-side job and item tables from the simulators' current state.
+    void build() {
+        if (sims.empty()) fail(BCSIM_ERR_INVALID_STATE, "empty batch");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        if (!stream) hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // frames in flight use the tables
+        drop_graph();
+        table_out = nullptr;
+        plan.clear();
+        renders.clear();
+        std::vector<ProjJob> pjobs;
+        std::vector<double> pcost;  // per-item cost estimate of each projection job
+        std::vector<ConvParams> cjobs;
+        std::vector<FillSeg> fills;
+        size_t offset = 0;
+        proj_lds = 0;
+        for (bcsim_sim* s : sims) {
+            s->prepare();
+            JobPlan jp{s, s->version, offset, (int)s->lines.size(), s->n_out};
+            plan.push_back(jp);
+            offset += (size_t)jp.lines * jp.samples;
+            const bool cplx = s->phase_delay;
+            const bool lut = s->profile == kProfileLut;
+            const bool shared = s->shared_origin();
+            size_t nonempty = 0;
+            for (auto& f : s->fixed) nonempty += f->n > 0;
+            for (auto& sp : s->splines) nonempty += sp->n > 0;
+            auto add_job = [&](const ProjCommon& c, const FixedSet* fs, const SplineSet* ss, const DevBuf& boxes,
+                               int nchunks, bool is_spline) {
+                ProjJob J{};
+                J.c = c;
+                J.cp = s->cull_params(c);
+                if (fs) J.fs = *fs;
+                if (ss) J.ss = *ss;
+                J.boxes = boxes.as<ChunkBox>();
+                J.bins = s->d_bins.as<float>();
+                J.g = s->tile_launch(nchunks);
+                J.g.block = 1024;
+                J.g.atomic = (J.g.slices > 1 || nonempty > 1) ? 1 : 0;
+                J.nchunks = nchunks;
+                J.flags = (s->arc ? kJobArc : 0) | (cplx ? kJobPhase : 0) | (lut ? kJobLut : 0) |
+                          (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0);
+                const int tiles = (c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
+                J.blocks_per_slice = tiles * J.g.windows;
+                proj_lds = std::max(proj_lds, tile_lds_bytes(J.g.window_bins, cplx, J.g.tile_lines, J.g.list_cap));
+                pjobs.push_back(J);
+                pcost.push_back((double)J.g.chunks_per_slice * J.g.tile_lines * J.g.window_bins);
+            };
+            for (auto& f : s->fixed) {
+                if (f->n == 0) continue;
+                const FixedSet fs{f->pts.as<float4>(), (int)f->n};
+                add_job(s->proj_common(false), &fs, nullptr, f->boxes, f->nchunks, false);
+            }
+            for (auto& sp : s->splines) {
+                if (sp->n == 0) continue;
+                const SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
+                                   sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
+                if (s->all_equal) {
+                    Render r{ss, {}, nullptr};
+                    s->host_basis_line0(*sp, r.basis);
+                    sp->rendered.ensure(sp->n * sizeof(float4));
+                    r.out = sp->rendered.as<float4>();
+                    renders.push_back(r);
+                    const FixedSet fs{r.out, (int)sp->n};
+                    add_job(s->proj_common(true), &fs, nullptr, sp->boxes, sp->nchunks, false);
+                } else {
+                    add_job(s->proj_common(true), nullptr, &ss, sp->boxes, sp->nchunks, true);
+                }
+            }
+            // bins that are accumulated (or not written at all) start from zero
+            bool single_store = false;
+            if (nonempty == 1) single_store = pjobs.back().g.atomic == 0;
+            if (!single_store)
+                fills.push_back(FillSeg{s->d_bins.as<float>(), s->lines.size() * (size_t)s->num_rf * (cplx ? 2 : 1)});
+            cjobs.push_back(s->conv_params(nullptr));  // iq pointer set per call (offset into the output)
+        }
+        total_out = offset;
+
+        // items, heaviest first
+        std::vector<int> porder(pjobs.size());
+        std::iota(porder.begin(), porder.end(), 0);
+        std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return pcost[a] > pcost[b]; });
+        std::vector<int2> pitems;
+        for (int j : porder) {
+            const int n = pjobs[j].blocks_per_slice * pjobs[j].g.slices;
+            for (int i = 0; i < n; i++) pitems.push_back(make_int2(j, i));
+        }
+        std::vector<int> corder(cjobs.size());
+        std::iota(corder.begin(), corder.end(), 0);
+        std::stable_sort(corder.begin(), corder.end(),
+                         [&](int a, int b) { return cjobs[a].fft_len > cjobs[b].fft_len; });
+        std::vector<int2> citems;
+        for (int j : corder)
+            for (int l = 0; l < cjobs[j].num_lines; l++) citems.push_back(make_int2(j, l));
+
+        auto upload = [&](DevBuf& d, const void* src, size_t bytes) {
+            d.ensure(std::max<size_t>(bytes, 16));
+            if (bytes) hip_check(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        };
+        upload(d_proj_jobs, pjobs.data(), pjobs.size() * sizeof(ProjJob));
+        upload(d_proj_items, pitems.data(), pitems.size() * sizeof(int2));
+        upload(d_fill, fills.data(), fills.size() * sizeof(FillSeg));
+        upload(d_conv_items, citems.data(), citems.size() * sizeof(int2));
+        d_conv_jobs.ensure(std::max<size_t>(cjobs.size() * sizeof(ConvParams), 16));
+        d_counters.ensure(2 * sizeof(int));
+        conv_host = cjobs;
+        n_proj_items = (int)pitems.size();
+        n_conv_items = (int)citems.size();
+        n_fill = (int)fills.size();
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        workgroups = std::max(1, prop.multiProcessorCount);
+        planned = true;
+    }
+    std::vector<ConvParams> conv_host;
+
+    // Point each job's IQ at its slice of `out` (device memory).  Waits for frames
+    // in flight first: they read the job table being rewritten.
+    float2* table_out = nullptr;
+    void set_output(float2* out) {
+        if (table_out == out) return;
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        std::vector<ConvParams> cj = conv_host;
+        for (size_t i = 0; i < cj.size(); i++) cj[i].iq = out + plan[i].out_offset;
+        hip_check(hipMemcpy(d_conv_jobs.p, cj.data(), cj.size() * sizeof(ConvParams), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        table_out = out;
+    }
+
+    void enqueue() {
+        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), 2, stream), "fill kernel");
+        for (const Render& r : renders)
+            hip_check(launch_render_spline(r.set, r.basis, r.out, stream), "render kernel");
+        hip_check(launch_project_batch(d_proj_jobs.as<ProjJob>(), d_proj_items.as<int2>(), n_proj_items,
+                                       d_counters.as<int>(), proj_lds, workgroups, stream),
+                  "batched projection kernel");
+        hip_check(launch_convolve_batch(d_conv_jobs.as<ConvParams>(), d_conv_items.as<int2>(), n_conv_items,
+                                        d_counters.as<int>() + 1, workgroups, stream),
+                  "batched convolution kernel");
+    }
+
+    void simulate_device(void* d_out, size_t capacity) {
+        if (stale()) build();
+        if (capacity < total_out) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
+        hipEvent_t a = nullptr, b = nullptr;
+        if (store_details) {
+            hip_check(hipEventCreate(&a), "hipEventCreate");
+            hip_check(hipEventCreate(&b), "hipEventCreate");
+            hip_check(hipEventRecord(a, stream), "hipEventRecord");
+        }
+        if (use_graph) {
+            if (!exec || graph_out != d_out) {
+                drop_graph();
+                set_output(static_cast<float2*>(d_out));
+                hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                try {
+                    enqueue();
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+                hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                graph_out = d_out;
+            }
+            hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+        } else {
+            set_output(static_cast<float2*>(d_out));
+            enqueue();
+        }
+        if (store_details) {
+            hip_check(hipEventRecord(b, stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+            debug["batch_frame_ms"].push_back(ms);
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
+};
+
+extern "C" {
+
+BCSIM_API bcsim_status bcsim_batch_create(bcsim_batch** out) {
+    return guarded([&] {
+        if (!out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output handle");
+        *out = new bcsim_batch();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_destroy(bcsim_batch* b) {
+    return guarded([&] { delete b; });
+}
+
+BCSIM_API bcsim_status bcsim_batch_set_parameter(bcsim_batch* b, const char* key, const char* value) {
+    return guarded([&] {
+        if (!b || !key || !value) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        const std::string k = key;
+        if (k == "use_graph") {
+            b->use_graph = parse_bool(value);
+            b->drop_graph();
+        } else if (k == "store_kernel_details") {
+            b->store_details = parse_bool(value);
+        } else {
+            fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal parameter name: '" + k + "'");
+        }
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_add(bcsim_batch* b, bcsim_sim* sim, int* job_index) {
+    return guarded([&] {
+        if (!b) fail(BCSIM_ERR_INVALID_ARGUMENT, "null batch handle");
+        b->add(sim);
+        if (job_index) *job_index = (int)b->sims.size() - 1;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_clear(bcsim_batch* b) {
+    return guarded([&] {
+        if (!b) fail(BCSIM_ERR_INVALID_ARGUMENT, "null batch handle");
+        b->sims.clear();
+        b->device = -1;
+        b->invalidate();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_get_layout(bcsim_batch* b, size_t* total, int* num_jobs, size_t* offsets,
+                                              int* lines, int* samples, int capacity) {
+    return guarded([&] {
+        if (!b) fail(BCSIM_ERR_INVALID_ARGUMENT, "null batch handle");
+        if (b->stale()) b->build();
+        if (total) *total = b->total_out;
+        if (num_jobs) *num_jobs = (int)b->plan.size();
+        for (int i = 0; i < (int)b->plan.size() && i < capacity; i++) {
+            if (offsets) offsets[i] = b->plan[i].out_offset;
+            if (lines) lines[i] = b->plan[i].lines;
+            if (samples) samples[i] = b->plan[i].samples;
+        }
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_simulate_device(bcsim_batch* b, void* d_out, size_t capacity) {
+    return guarded([&] {
+        if (!b || !d_out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        b->simulate_device(d_out, capacity);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_simulate(bcsim_batch* b, float* out, size_t capacity) {
+    return guarded([&] {
+        if (!b || !out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        if (b->stale()) b->build();
+        if (capacity < b->total_out) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
+        DevBuf iq;
+        iq.ensure(std::max<size_t>(b->total_out, 1) * sizeof(float2));
+        const bool graph = b->use_graph;
+        b->use_graph = false;  // a temporary output buffer is not worth a capture
+        try {
+            b->simulate_device(iq.p, b->total_out);
+        } catch (...) {
+            b->use_graph = graph;
+            throw;
+        }
+        b->use_graph = graph;
+        hip_check(hipMemcpyAsync(out, iq.p, b->total_out * sizeof(float2), hipMemcpyDeviceToHost, b->stream),
+                  "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_synchronize(bcsim_batch* b) {
+    return guarded([&] {
+        if (!b) fail(BCSIM_ERR_INVALID_ARGUMENT, "null batch handle");
+        if (b->stream) hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    });
+}
+
+BCSIM_API bcsim_status bcsim_batch_get_debug_data(bcsim_batch* b, const char* key, double* out, size_t capacity,
+                                                  size_t* count) {
+    return guarded([&] {
+        if (!b || !key) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        auto it = b->debug.find(key);
+        if (it == b->debug.end()) fail(BCSIM_ERR_INVALID_ARGUMENT, std::string("invalid debug data key: ") + key);
+        if (count) *count = it->second.size();
+        const size_t n = std::min(capacity, it->second.size());
+        if (out && n) std::memcpy(out, it->second.data(), n * sizeof(double));
+    });
+}
+
+}  // extern "C"

@@ -110,15 +110,17 @@ __device__ __forceinline__ void emit(const ProjCommon& c, float* lbins, int b0, 
     add_bin<PHASE>(c, lbins, lb, bin, ti, w);
 }
 
+// One (tile, window, slice) block of work: everything a workgroup of the tile
+// kernel does.  `wg` = tile * windows + window; `lds` is the dynamic LDS and
+// `list_n` a workgroup-shared counter.
 template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
-__global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullParams cp, FixedSet fs, SplineSet ss,
-                                                            const ChunkBox* __restrict__ boxes, int nchunks,
-                                                            float* __restrict__ out, TileLaunch g) {
-    extern __shared__ float lds[];
-    __shared__ int list_n;
+__device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams& cp, const FixedSet& fs,
+                                           const SplineSet& ss, const ChunkBox* __restrict__ boxes, int nchunks,
+                                           float* __restrict__ out, const TileLaunch& g, int wg, int slice,
+                                           float* lds, int& list_n) {
     constexpr int E = PHASE ? 2 : 1;
-    const int window = blockIdx.x % g.windows;
-    const int tile = blockIdx.x / g.windows;
+    const int window = wg % g.windows;
+    const int tile = wg / g.windows;
     const int line0 = tile * g.tile_lines;
     const int nl = min(g.tile_lines, c.num_lines - line0);
     const int b0 = window * g.window_bins;
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullPa
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
-    const int cbeg = blockIdx.y * g.chunks_per_slice;
+    const int cbeg = slice * g.chunks_per_slice;
     const int cend = min(nchunks, cbeg + g.chunks_per_slice);
     const int N = SPLINE ? ss.n : fs.n;
 
@@ -252,6 +254,81 @@ __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullPa
             *dst = v;
         }
     }
+}
+
+template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
+__global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullParams cp, FixedSet fs, SplineSet ss,
+                                                            const ChunkBox* __restrict__ boxes, int nchunks,
+                                                            float* __restrict__ out, TileLaunch g) {
+    extern __shared__ float lds[];
+    __shared__ int list_n;
+    tile_block<ARC, PHASE, LUT, SPLINE, SHARED>(c, cp, fs, ss, boxes, nchunks, out, g, blockIdx.x, blockIdx.y, lds,
+                                                list_n);
+}
+
+// Persistent projection over many jobs (simulators x datasets): every
+// workgroup takes (job, block) items from a global counter until the list is
+// exhausted, and runs the block with the job's own kernel variant.
+template <bool ARC, bool PHASE, bool LUT>
+__device__ __forceinline__ void tile_block_job(const ProjJob& J, int wg, int slice, float* lds, int& list_n) {
+    if (J.flags & kJobSpline)
+        tile_block<ARC, PHASE, LUT, true, false>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg, slice,
+                                                 lds, list_n);
+    else if (ARC && (J.flags & kJobShared))
+        tile_block<ARC, PHASE, LUT, false, true>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg, slice,
+                                                 lds, list_n);
+    else
+        tile_block<ARC, PHASE, LUT, false, false>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg, slice,
+                                                  lds, list_n);
+}
+
+__global__ __launch_bounds__(1024) void project_batch_kernel(const ProjJob* __restrict__ jobs,
+                                                             const int2* __restrict__ items, int n_items,
+                                                             int* __restrict__ counter) {
+    extern __shared__ float lds[];
+    __shared__ int list_n;
+    __shared__ int item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = item_s;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int2 w = items[it];  // (job, block within the job)
+        const ProjJob J = jobs[w.x];
+        const int per_slice = J.blocks_per_slice;
+        const int wg = w.y % per_slice, slice = w.y / per_slice;
+        const bool arc = J.flags & kJobArc, phase = J.flags & kJobPhase, lut = J.flags & kJobLut;
+        if (arc) {
+            if (phase) {
+                if (lut) tile_block_job<true, true, true>(J, wg, slice, lds, list_n);
+                else tile_block_job<true, true, false>(J, wg, slice, lds, list_n);
+            } else {
+                if (lut) tile_block_job<true, false, true>(J, wg, slice, lds, list_n);
+                else tile_block_job<true, false, false>(J, wg, slice, lds, list_n);
+            }
+        } else {
+            if (phase) {
+                if (lut) tile_block_job<false, true, true>(J, wg, slice, lds, list_n);
+                else tile_block_job<false, true, false>(J, wg, slice, lds, list_n);
+            } else {
+                if (lut) tile_block_job<false, false, true>(J, wg, slice, lds, list_n);
+                else tile_block_job<false, false, false>(J, wg, slice, lds, list_n);
+            }
+        }
+        __syncthreads();  // the next item reuses the LDS
+    }
+}
+
+hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds,
+                                int workgroups, hipStream_t stream) {
+    if (n_items == 0) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute((const void*)project_batch_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(project_batch_kernel, dim3(workgroups), dim3(1024), lds, stream, jobs, items, n_items,
+                       counter);
+    return hipGetLastError();
 }
 
 size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap) {

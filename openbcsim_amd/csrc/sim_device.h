@@ -146,6 +146,29 @@ struct TileLaunch {
     int list_cap;             // chunk list entries in LDS (multiple of 64)
 };
 
+// One projection job of a batch: a (simulator, dataset) pair with everything its
+// tile kernel would receive as arguments.
+enum JobFlags { kJobArc = 1, kJobPhase = 2, kJobLut = 4, kJobSpline = 8, kJobShared = 16 };
+
+struct ProjJob {
+    ProjCommon c;
+    CullParams cp;
+    FixedSet fs;
+    SplineSet ss;
+    const ChunkBox* boxes;
+    float* bins;
+    TileLaunch g;
+    int nchunks;
+    int flags;
+    int blocks_per_slice;     // tiles * windows
+};
+
+// A region of floats to zero before a batched frame.
+struct FillSeg {
+    float* p;
+    size_t n;
+};
+
 // Launchers (sim_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_fill_zero(float* p, size_t n_floats, hipStream_t stream);
 hipError_t launch_project_fixed(const ProjCommon& c, const FixedSet& s, void* bins, bool complex_bins,
@@ -166,5 +189,15 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams& cp, const 
                                bool arc, bool phase, bool lut, bool shared_origin, const TileLaunch& cfg,
                                hipStream_t stream);
 size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap);
+
+// Batched frames (one launch each for all jobs of a batch): persistent
+// workgroups take items from `counter` (reset by launch_fill_segments).
+hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds,
+                                int workgroups, hipStream_t stream);
+hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int n_items, int* counter,
+                                 int workgroups, hipStream_t stream);
+hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, int n_counters,
+                                hipStream_t stream);
+size_t convolve_batch_lds_bytes();
 
 }  // namespace bcsim_hip

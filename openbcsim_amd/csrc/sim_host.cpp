@@ -15,1104 +15,23 @@
 // to [0, num_rf), demodulation phase referenced to the extracted sample,
 // trilinear LUT sampling on grid points); the error behaviour follows
 // GpuAlgorithm (what it validates, and with which message).
-#include <hip/hip_runtime.h>
+#include "sim_state.h"
 
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <memory>
-#include <stdexcept>
-#include <string>
-#include <vector>
-
-#include "../../include/bcsim_c.h"
-#include "sim_device.h"
-
-using namespace bcsim_hip;
-
-namespace {
-
-thread_local std::string g_last_error;
-
-struct BcsimError : std::runtime_error {
-    int code;
-    BcsimError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-[[noreturn]] void fail(int code, const std::string& msg) { throw BcsimError(code, msg); }
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) {
-        const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? BCSIM_ERR_OUT_OF_MEMORY
-                                                                                      : BCSIM_ERR_DEVICE;
-        fail(code, std::string(what) + ": " + hipGetErrorString(e));
-    }
+namespace bcsim_hip {
+std::string& last_error() {
+    thread_local std::string msg;
+    return msg;
 }
+}  // namespace bcsim_hip
 
-// Device buffer owned by a simulator.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void ensure(size_t n) {
-        if (n <= bytes && p) return;
-        release();
-        if (n == 0) return;
-        hip_check(hipMalloc(&p, n), "hipMalloc");
-        bytes = n;
-    }
-    template <typename T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
-// Datasets are stored in Morton (z-curve) order of position, in chunks of
-// kChunk scatterers with one bounding box each, so the tile kernel can skip
-// whole chunks that lie outside a beam.  The order of the scatterers does not
-// change the frame beyond float summation order.
-struct FixedDataset {
-    DevBuf pts;    // float4[n] (x, y, z, amplitude), Morton order
-    DevBuf boxes;  // ChunkBox[nchunks]
-    size_t n = 0;
-    int nchunks = 0;
-};
-
-struct SplineDataset {
-    int degree = 0;
-    int num_cp = 0;
-    size_t n = 0;
-    int nchunks = 0;
-    std::vector<float> knots;
-    DevBuf cx, cy, cz, amp;  // [num_cp][n] x3, [n], Morton order of the control-point centroid
-    DevBuf boxes;            // ChunkBox[nchunks] over all control points of the chunk
-    DevBuf basis;            // SplineLineBasis[num_lines]
-    DevBuf rendered;         // float4[n], all-timestamps-equal path (boxes: `boxes`, since a
-                             // rendered point lies in the box of its control points)
-};
-
-// 10 bits of each coordinate interleaved into a 30-bit z-curve key.
-uint32_t spread10(uint32_t x) {
-    x &= 0x3ffu;
-    x = (x | (x << 16)) & 0x030000ffu;
-    x = (x | (x << 8)) & 0x0300f00fu;
-    x = (x | (x << 4)) & 0x030c30c3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
-
-bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
-
-// Permutation that puts the points pos[i*stride .. +2] in z-curve order over
-// their bounding box; non-finite points go last.
-std::vector<uint32_t> morton_order(const float* pos, size_t n, size_t stride) {
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (size_t i = 0; i < n; i++) {
-        const float* p = pos + i * stride;
-        if (!finite3(p)) continue;
-        for (int a = 0; a < 3; a++) {
-            lo[a] = std::min(lo[a], p[a]);
-            hi[a] = std::max(hi[a], p[a]);
-        }
-    }
-    std::vector<std::pair<uint32_t, uint32_t>> keyed(n);
-    for (size_t i = 0; i < n; i++) {
-        const float* p = pos + i * stride;
-        uint32_t key = 0xffffffffu;
-        if (finite3(p)) {
-            key = 0;
-            for (int a = 0; a < 3; a++) {
-                const double span = (double)hi[a] - (double)lo[a];
-                double q = span > 0 ? ((double)p[a] - (double)lo[a]) / span * 1024.0 : 0.0;
-                const uint32_t qi = (uint32_t)std::min(1023.0, std::max(0.0, q));
-                key |= spread10(qi) << a;
-            }
-        }
-        keyed[i] = {key, (uint32_t)i};
-    }
-    std::sort(keyed.begin(), keyed.end());
-    std::vector<uint32_t> perm(n);
-    for (size_t i = 0; i < n; i++) perm[i] = keyed[i].second;
-    return perm;
-}
-
-// Box of a set of points, padded by a relative 1e-6 so that rounding of the
-// centre and half extent never shrinks it.
-struct BoxAcc {
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    bool any = false;
-    void add(const float* p) {
-        if (!finite3(p)) return;
-        any = true;
-        for (int a = 0; a < 3; a++) {
-            lo[a] = std::min(lo[a], p[a]);
-            hi[a] = std::max(hi[a], p[a]);
-        }
-    }
-    ChunkBox box() const {
-        ChunkBox b{0.f, 0.f, 0.f, 0.f, -1.f, -1.f, -1.f, 0.f};
-        if (!any) return b;
-        float c[3], h[3];
-        for (int a = 0; a < 3; a++) {
-            const double cd = 0.5 * ((double)lo[a] + (double)hi[a]);
-            c[a] = (float)cd;
-            const double hd = std::max((double)hi[a] - (double)c[a], (double)c[a] - (double)lo[a]);
-            h[a] = (float)(hd * (1.0 + 1e-6) + 1e-30);
-        }
-        b = ChunkBox{c[0], c[1], c[2], 0.f, h[0], h[1], h[2], 0.f};
-        return b;
-    }
-};
-
-// Vector3D<float>::dot evaluated left to right (vector3.hpp:86-88).
-float dot3(const float* a, const float* b) {
-    float t = a[0] * b[0];
-    t = t + a[1] * b[1];
-    t = t + a[2] * b[2];
-    return t;
-}
-
-// bspline_storve::bsplineBasis<float> (bspline.hpp:45-70): the float recursion
-// whose values the reference multiplies with the control points.
-float special_div(float num, float den) {
-    if (std::fabs((double)num) < 1e-6 && std::fabs((double)den) < 1e-6) return 0.0f;
-    return num / den;
-}
-
-float bspline_basis(int j, int p, float x, const std::vector<float>& knots) {
-    if (p == 0) return (x >= knots[j] && x < knots[j + 1]) ? 1.0f : 0.0f;
-    const float left = special_div((x - knots[j]) * bspline_basis(j, p - 1, x, knots), knots[j + p] - knots[j]);
-    const float right = special_div((knots[j + 1 + p] - x) * bspline_basis(j + 1, p - 1, x, knots),
-                                    knots[j + 1 + p] - knots[j + 1]);
-    return left + right;
-}
-
-// bspline_storve::compute_knot_interval (bspline.hpp:75-82)
-int knot_interval(const std::vector<float>& knots, float t) {
-    for (int i = 0; i + 1 < (int)knots.size(); i++)
-        if (t >= knots[i] && t < knots[i + 1]) return i;
-    fail(BCSIM_ERR_INVALID_ARGUMENT, "compute_knot_interval : could not determine knot interval");
-}
-
-bool parse_bool(const std::string& v) {
-    if (v == "on" || v == "true") return true;
-    if (v == "off" || v == "false") return false;
-    fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid boolean value");
-}
-
-int parse_int(const std::string& v) {
-    try {
-        size_t pos = 0;
-        const int x = std::stoi(v, &pos);
-        return x;
-    } catch (...) {
-        fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid integer value: '" + v + "'");
-    }
-}
-
-float parse_float(const std::string& v) {
-    try {
-        return std::stof(v);
-    } catch (...) {
-        fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid number: '" + v + "'");
-    }
-}
-
-}  // namespace
-
-// -------------------------------------------------------------------------------------------
-
-struct bcsim_sim {
-    // parameters (BaseAlgorithm.cpp:36-46, GpuAlgorithm.cpp:46-60 defaults)
-    int verbose = 0;
-    float sound_speed = 1540.0f;
-    float noise_amplitude = 0.0f;
-    uint32_t noise_seed = 0;
-    uint32_t noise_counter = 0;
-    bool arc = true;
-    int decimation = 1;
-    bool phase_delay = false;
-    int device = 0;
-    bool can_change_device = true;
-    int num_streams = 2;
-    int threads_per_block = 128;
-    bool store_details = false;
-    bool direct_mode = false;  // projection_mode: "tiled" (culled tile kernel) or "direct"
-    // launch shape of the tile kernel; 0 = chosen from the frame shape
-    struct {
-        int windows = 0, lines = 0, block = 0, slices = 0;
-    } tune;
-
-    hipStream_t stream = nullptr;
-    bcsim_log_fn log_fn = nullptr;
-    void* log_user = nullptr;
-
-    // excitation
-    bool have_exc = false;
-    std::vector<float> exc;
-    int center = 0;
-    float fs = 0.0f;
-    float f_demod = 0.0f;
-
-    // scan
-    bool have_scan = false;
-    std::vector<LineGeom> lines;
-    float line_length = 0.0f;
-    bool all_equal = false;
-
-    // phantom
-    std::vector<std::unique_ptr<FixedDataset>> fixed;
-    std::vector<std::unique_ptr<SplineDataset>> splines;
-
-    // profile
-    int profile = -1;
-    float sigma_l = 0.f, sigma_e = 0.f;
-    DevBuf lut;  // float4 quads [n_rad][n_lat][n_ele]
-    int lut_nr = 0, lut_nl = 0, lut_ne = 0;
-    float lut_r0 = 0, lut_r1 = 0, lut_l0 = 0, lut_l1 = 0, lut_e0 = 0, lut_e1 = 0;
-
-    // derived device tables
-    bool dirty_scan = true, dirty_conv = true, dirty_basis = true;
-    int num_rf = 0, fft_len = 0, log2_len = 0, n_out = 0;
-    DevBuf d_lines, d_H, d_demod, d_tw, d_bins, d_iq;
-
-    std::map<std::string, std::vector<double>> debug;
-
-    ~bcsim_sim() {
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
-
-    void log(int level, const std::string& msg) {
-        if (log_fn) log_fn(level, msg.c_str(), log_user);
-    }
-
-    void use_device() {
-        hip_check(hipSetDevice(device), "hipSetDevice");
-        if (!stream) hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-        can_change_device = false;
-    }
-
-    // ---------------------------------------------------------------- parameters
-    void set_parameter(const std::string& key, const std::string& value) {
-        if (key == "verbose") {
-            verbose = parse_int(value);
-        } else if (key == "sound_speed") {
-            const float c = parse_float(value);
-            if (c <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal sound speed");
-            sound_speed = c;
-            dirty_scan = dirty_conv = true;
-        } else if (key == "noise_amplitude") {
-            noise_amplitude = parse_float(value);
-        } else if (key == "noise_seed") {
-            noise_seed = (uint32_t)std::stoul(value);
-            noise_counter = 0;
-        } else if (key == "use_arc_projection") {
-            arc = parse_bool(value);
-        } else if (key == "radial_decimation") {
-            const int d = parse_int(value);
-            if (d <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal radial decimation value");
-            decimation = d;
-            dirty_conv = true;
-        } else if (key == "phase_delay") {
-            phase_delay = parse_bool(value);
-        } else if (key == "gpu_device") {
-            if (!can_change_device) fail(BCSIM_ERR_INVALID_STATE, "cannot change CUDA device now");
-            int count = 0;
-            hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
-            const int d = parse_int(value);
-            if (d < 0 || d >= count) fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal device number");
-            device = d;
-        } else if (key == "cuda_streams") {
-            const int n = parse_int(value);
-            if (n > 2) fail(BCSIM_ERR_INVALID_ARGUMENT, "number of CUDA streams exceeds MAX_NUM_CUDA_STREAMS");
-            if (n <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "number of CUDA streams must be more than zero");
-            num_streams = n;  // a frame is one stream-ordered launch sequence; kept for compatibility
-        } else if (key == "threads_per_block") {
-            const int n = parse_int(value);
-            if (n <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid number of threads per block");
-            threads_per_block = n;  // launch shapes come from the tuner; kept for compatibility
-        } else if (key == "store_kernel_details") {
-            store_details = parse_bool(value);
-        } else if (key == "projection_mode") {
-            if (value == "tiled") direct_mode = false;
-            else if (value == "direct") direct_mode = true;
-            else fail(BCSIM_ERR_INVALID_ARGUMENT, "projection_mode must be 'tiled' or 'direct'");
-        } else if (key == "tile_windows" || key == "tile_lines" || key == "tile_block" || key == "tile_slices") {
-            const int v = parse_int(value);
-            if (v < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, key + " must be >= 0 (0 = automatic)");
-            if (key == "tile_lines" && v > 32) fail(BCSIM_ERR_INVALID_ARGUMENT, "tile_lines must be <= 32");
-            if (key == "tile_block" && (v % 64 != 0 || v > 1024))
-                fail(BCSIM_ERR_INVALID_ARGUMENT, "tile_block must be a multiple of 64, at most 1024");
-            if (key == "tile_windows") tune.windows = v;
-            if (key == "tile_lines") tune.lines = v;
-            if (key == "tile_block") tune.block = v;
-            if (key == "tile_slices") tune.slices = v;
-        } else {
-            fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal parameter name: '" + key + "'");
-        }
-    }
-
-    std::string get_parameter(const std::string& key) {
-        if (key == "num_cuda_devices") {
-            int count = 0;
-            hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
-            return std::to_string(count);
-        }
-        if (key == "cur_device_name") {
-            hipDeviceProp_t prop;
-            hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-            return std::string(prop.name) + " (" + prop.gcnArchName + ")";
-        }
-        if (key == "verbose") return std::to_string(verbose);
-        if (key == "sound_speed") return std::to_string(sound_speed);
-        if (key == "noise_amplitude") return std::to_string(noise_amplitude);
-        if (key == "noise_seed") return std::to_string(noise_seed);
-        if (key == "use_arc_projection") return arc ? "on" : "off";
-        if (key == "radial_decimation") return std::to_string(decimation);
-        if (key == "phase_delay") return phase_delay ? "on" : "off";
-        if (key == "gpu_device") return std::to_string(device);
-        if (key == "cuda_streams") return std::to_string(num_streams);
-        if (key == "threads_per_block") return std::to_string(threads_per_block);
-        if (key == "store_kernel_details") return store_details ? "on" : "off";
-        if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
-        if (key == "tile_windows") return std::to_string(tune.windows);
-        if (key == "tile_lines") return std::to_string(tune.lines);
-        if (key == "tile_block") return std::to_string(tune.block);
-        if (key == "tile_slices") return std::to_string(tune.slices);
-        fail(BCSIM_ERR_INVALID_ARGUMENT, "Illegal key: " + key);
-    }
-
-    // ---------------------------------------------------------------- configuration
-    void set_excitation(const float* samples, int n, int center_index, float fs_, float fd) {
-        if (!samples || n <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "excitation must have at least one sample");
-        if (center_index < 0 || center_index >= n)
-            fail(BCSIM_ERR_INVALID_ARGUMENT, "excitation center_index must lie inside the excitation");
-        if (!(fs_ > 0)) fail(BCSIM_ERR_INVALID_ARGUMENT, "sampling frequency must be positive");
-        can_change_device = false;
-        exc.assign(samples, samples + n);
-        center = center_index;
-        fs = fs_;
-        f_demod = fd;
-        have_exc = true;
-        dirty_scan = dirty_conv = dirty_basis = true;
-        log(1, "Number of excitation samples: " + std::to_string(n));
-    }
-
-    void set_scan_sequence(int L, const float* o, const float* d, const float* lat, const float* t,
-                           float length, bool equal) {
-        if (!have_exc) fail(BCSIM_ERR_INVALID_STATE, "Excitation must be configured before scan sequence");
-        if (L < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "negative number of scan lines");
-        if (L > 0 && (!o || !d || !lat || !t)) fail(BCSIM_ERR_INVALID_ARGUMENT, "null scan sequence array");
-        std::vector<LineGeom> g(L);
-        for (int i = 0; i < L; i++) {
-            LineGeom& q = g[i];
-            const float* oi = o + 3 * i;
-            const float* di = d + 3 * i;
-            const float* li = lat + 3 * i;
-            q = LineGeom{oi[0], oi[1], oi[2], t[i], di[0], di[1], di[2], 0.f, li[0], li[1], li[2], 0.f,
-                         0.f, 0.f, 0.f, 0.f};
-            // elevational = lateral x direction (ScanSequence.cpp:40, vector3.hpp:96-101)
-            q.ex = li[1] * di[2] - di[1] * li[2];
-            q.ey = di[0] * li[2] - li[0] * di[2];
-            q.ez = li[0] * di[1] - di[0] * li[1];
-            const float e[3] = {q.ex, q.ey, q.ez};
-            const float th = 1e-4f;  // ScanSequence.cpp:52-70
-            bool ok = std::fabs(dot3(di, li)) < th && std::fabs(dot3(li, e)) < th && std::fabs(dot3(e, di)) < th;
-            ok = ok && std::fabs((double)std::sqrt(dot3(di, di)) - 1.0) < th &&
-                 std::fabs((double)std::sqrt(dot3(li, li)) - 1.0) < th &&
-                 std::fabs((double)std::sqrt(dot3(e, e)) - 1.0) < th;
-            if (!ok) fail(BCSIM_ERR_INVALID_ARGUMENT, "Invalid Scanline: Not orthogonal unit vectors (line " +
-                                                           std::to_string(i) + ")");
-        }
-        lines.swap(g);
-        line_length = length;
-        all_equal = equal;
-        have_scan = true;
-        can_change_device = false;
-        dirty_scan = dirty_conv = dirty_basis = true;
-    }
-
-    void clear_fixed() { fixed.clear(); }
-
-    void add_fixed(const float* xyza, size_t n) {
-        if (n > 0 && !xyza) fail(BCSIM_ERR_INVALID_ARGUMENT, "null scatterer array");
-        if (n > (size_t)0x7fffffff) fail(BCSIM_ERR_INVALID_ARGUMENT, "too many scatterers in one dataset");
-        use_device();
-        auto ds = std::make_unique<FixedDataset>();
-        ds->n = n;
-        ds->nchunks = (int)((n + kChunk - 1) / kChunk);
-        const std::vector<uint32_t> perm = morton_order(xyza, n, 4);
-        std::vector<float> sorted(n * 4);
-        for (size_t i = 0; i < n; i++) std::memcpy(&sorted[4 * i], xyza + 4 * (size_t)perm[i], 4 * sizeof(float));
-        std::vector<ChunkBox> boxes(ds->nchunks);
-        for (int c = 0; c < ds->nchunks; c++) {
-            BoxAcc acc;
-            for (size_t i = (size_t)c * kChunk; i < std::min(n, (size_t)(c + 1) * kChunk); i++) acc.add(&sorted[4 * i]);
-            boxes[c] = acc.box();
-        }
-        ds->pts.ensure(n * sizeof(float4));
-        ds->boxes.ensure(boxes.size() * sizeof(ChunkBox));
-        if (n) {
-            hip_check(hipMemcpy(ds->pts.p, sorted.data(), n * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(hipMemcpy(ds->boxes.p, boxes.data(), boxes.size() * sizeof(ChunkBox), hipMemcpyHostToDevice),
-                      "hipMemcpy");
-        }
-        fixed.push_back(std::move(ds));
-        if (verbose) log(1, "Number of fixed scatterers: " + std::to_string(total_fixed()));
-    }
-
-    void clear_spline() { splines.clear(); }
-
-    void add_spline(int degree, const float* knots, int n_knots, const float* cps, const float* amps, size_t n,
-                    int num_cp) {
-        if (degree < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "spline degree must be non-negative");
-        if (degree > kMaxSplineDegree) fail(BCSIM_ERR_INVALID_ARGUMENT, "maximum spline degree supported is 4");
-        if (num_cp <= degree) fail(BCSIM_ERR_INVALID_ARGUMENT, "too few spline control points for given degree");
-        if (n_knots != num_cp + degree + 1)
-            fail(BCSIM_ERR_INVALID_ARGUMENT, "mismatch in number of nodes, degree and control points");
-        if (!knots || (n > 0 && (!cps || !amps))) fail(BCSIM_ERR_INVALID_ARGUMENT, "null spline array");
-        if (n > (size_t)0x7fffffff) fail(BCSIM_ERR_INVALID_ARGUMENT, "too many scatterers in one dataset");
-        use_device();
-        auto ds = std::make_unique<SplineDataset>();
-        ds->degree = degree;
-        ds->num_cp = num_cp;
-        ds->n = n;
-        ds->knots.assign(knots, knots + n_knots);
-        ds->nchunks = (int)((n + kChunk - 1) / kChunk);
-        // order by the centroid of each spline's control points
-        std::vector<float> centroid(n * 3);
-        for (size_t s = 0; s < n; s++) {
-            double acc[3] = {0, 0, 0};
-            for (int i = 0; i < num_cp; i++)
-                for (int a = 0; a < 3; a++) acc[a] += cps[(s * num_cp + i) * 3 + a];
-            for (int a = 0; a < 3; a++) centroid[3 * s + a] = (float)(acc[a] / num_cp);
-        }
-        const std::vector<uint32_t> perm = morton_order(centroid.data(), n, 3);
-        // [n][num_cp][3] -> three control-point-major planes [num_cp][n], sorted
-        std::vector<float> px((size_t)num_cp * n), py((size_t)num_cp * n), pz((size_t)num_cp * n), am(n);
-        std::vector<ChunkBox> boxes(ds->nchunks);
-        for (int c = 0; c < ds->nchunks; c++) {
-            BoxAcc acc;
-            for (size_t s = (size_t)c * kChunk; s < std::min(n, (size_t)(c + 1) * kChunk); s++) {
-                const size_t src = perm[s];
-                am[s] = amps[src];
-                for (int i = 0; i < num_cp; i++) {
-                    const float* cp = cps + (src * num_cp + i) * 3;
-                    px[(size_t)i * n + s] = cp[0];
-                    py[(size_t)i * n + s] = cp[1];
-                    pz[(size_t)i * n + s] = cp[2];
-                    acc.add(cp);
-                }
-            }
-            boxes[c] = acc.box();
-        }
-        const size_t b = px.size() * sizeof(float);
-        ds->cx.ensure(b);
-        ds->cy.ensure(b);
-        ds->cz.ensure(b);
-        ds->amp.ensure(n * sizeof(float));
-        ds->boxes.ensure(boxes.size() * sizeof(ChunkBox));
-        if (n) {
-            hip_check(hipMemcpy(ds->cx.p, px.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(hipMemcpy(ds->cy.p, py.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(hipMemcpy(ds->cz.p, pz.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(hipMemcpy(ds->amp.p, am.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(hipMemcpy(ds->boxes.p, boxes.data(), boxes.size() * sizeof(ChunkBox), hipMemcpyHostToDevice),
-                      "hipMemcpy");
-        }
-        splines.push_back(std::move(ds));
-        dirty_basis = true;
-        if (verbose) log(1, "Number of spline scatterers: " + std::to_string(total_spline()));
-    }
-
-    void set_gaussian(float sl, float se) {
-        profile = kProfileGaussian;
-        sigma_l = sl;
-        sigma_e = se;
-    }
-
-    void set_lut(float r0, float r1, float l0, float l1, float e0, float e1, const float* samples, int nr,
-                 int nl, int ne) {
-        // LUTBeamProfile constructor (BeamProfile.cpp:58-72)
-        if (nr <= 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "Too few radial samples");
-        if (nl <= 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "Too few lateral samples");
-        if (ne <= 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "Too few elevational samples");
-        if (!samples) fail(BCSIM_ERR_INVALID_ARGUMENT, "null LUT samples");
-        use_device();
-        // quads of the (l, e) neighbourhood, see lut_weight in proj_math.h
-        const size_t cells = (size_t)nr * nl * ne;
-        std::vector<float4> quad(cells);
-        auto at = [&](int r, int l, int e) -> float {
-            return (l < nl && e < ne) ? samples[((size_t)r * nl + l) * ne + e] : 0.0f;
-        };
-        for (int r = 0; r < nr; r++)
-            for (int l = 0; l < nl; l++)
-                for (int e = 0; e < ne; e++)
-                    quad[((size_t)r * nl + l) * ne + e] =
-                        make_float4(at(r, l, e), at(r, l, e + 1), at(r, l + 1, e), at(r, l + 1, e + 1));
-        lut.ensure(cells * sizeof(float4));
-        hip_check(hipMemcpy(lut.p, quad.data(), cells * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
-        lut_nr = nr;
-        lut_nl = nl;
-        lut_ne = ne;
-        lut_r0 = r0; lut_r1 = r1;
-        lut_l0 = l0; lut_l1 = l1;
-        lut_e0 = e0; lut_e1 = e1;
-        profile = kProfileLut;
-    }
-
-    size_t total_fixed() const {
-        size_t s = 0;
-        for (auto& f : fixed) s += f->n;
-        return s;
-    }
-    size_t total_spline() const {
-        size_t s = 0;
-        for (auto& f : splines) s += f->n;
-        return s;
-    }
-
-    // ---------------------------------------------------------------- derived tables
-    int compute_num_rf() const {
-        // compute_num_rf_samples<float> (common_utils.hpp:37-41)
-        const float max_time = (float)(2.0 * (double)line_length / (double)sound_speed);
-        const float prod = fs * max_time;
-        const double v = std::floor((double)prod + 0.5);
-        if (!(v >= 0.0 && v < 1e9)) fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid line length");
-        return (int)v;
-    }
-
-    void require_configured() {
-        if (!have_exc) fail(BCSIM_ERR_INVALID_STATE, "Excitation not configured.");
-        if (!have_scan) fail(BCSIM_ERR_INVALID_STATE, "Scan sequence not configured.");
-    }
-
-    void update_shape() {
-        require_configured();
-        const int nrf = compute_num_rf();
-        if (nrf < 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "line length gives no RF samples");
-        size_t need = (size_t)nrf + exc.size() - 1;
-        size_t N = 1;
-        while (N < need) N <<= 1;  // next_power_of_two (fft.cpp:80-82)
-        if (N > (size_t)kMaxFftLen) {
-            log(3, "num_rf_samples required: " + std::to_string(nrf));
-            fail(BCSIM_ERR_INVALID_ARGUMENT,
-                 "Too many RF samples required: FFT length " + std::to_string(N) + " exceeds " +
-                     std::to_string(kMaxFftLen));
-        }
-        num_rf = nrf;
-        fft_len = (int)N;
-        log2_len = 0;
-        while ((1 << log2_len) < fft_len) log2_len++;
-        n_out = (num_rf + decimation - 1) / decimation;
-    }
-
-    void upload_conv_tables() {
-        const int N = fft_len;
-        // H = mask * FFT_N(excitation) / N in double (BeamConvolver.cpp:77-91, the 1/N of
-        // GpuAlgorithm.cpp:406), by direct summation over the excitation's few samples.
-        std::vector<float2> H(N);
-        const double two_pi = 2.0 * M_PI;
-        for (int k = 0; k < N; k++) {
-            double mask = 0.0;
-            if (N == 1) mask = 1.0;
-            else if (k == 0) mask = 1.0;
-            else if (k < N / 2) mask = 2.0;
-            else if (k == N / 2) mask = 1.0;
-            double re = 0, im = 0;
-            if (mask != 0.0) {
-                for (size_t n = 0; n < exc.size(); n++) {
-                    const long long kn = ((long long)k * (long long)n) % N;
-                    const double a = -two_pi * (double)kn / (double)N;
-                    re += (double)exc[n] * std::cos(a);
-                    im += (double)exc[n] * std::sin(a);
-                }
-            }
-            H[k] = make_float2((float)(mask * re / N), (float)(mask * im / N));
-        }
-        // demodulation: exp(-2 pi i nfd n), n = j*D, with nfd the reference's float quotient
-        // (CpuAlgorithm.cpp:353-358); the cycle count nfd*n is exact in double.
-        std::vector<float2> dem(n_out);
-        const double nfd = (double)(f_demod / fs);
-        for (int j = 0; j < n_out; j++) {
-            double cyc = nfd * (double)((long long)j * decimation);
-            cyc -= std::floor(cyc);
-            const double a = -two_pi * cyc;
-            dem[j] = make_float2((float)std::cos(a), (float)std::sin(a));
-        }
-        // two-level twiddles exp(-2 pi i k / N): coarse[k >> b] * fine[k & (2^b - 1)]
-        const int b = log2_len / 2;
-        const int nc = N >> b, nf = 1 << b;
-        std::vector<float2> tw(nc + nf);
-        for (int i = 0; i < nc; i++) {
-            const double a = -two_pi * (double)((long long)i << b) / (double)N;
-            tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
-        }
-        for (int i = 0; i < nf; i++) {
-            const double a = -two_pi * (double)i / (double)N;
-            tw[nc + i] = make_float2((float)std::cos(a), (float)std::sin(a));
-        }
-        d_H.ensure(sizeof(float2) * N);
-        d_demod.ensure(sizeof(float2) * std::max(n_out, 1));
-        d_tw.ensure(sizeof(float2) * tw.size());
-        hip_check(hipMemcpy(d_H.p, H.data(), sizeof(float2) * N, hipMemcpyHostToDevice), "hipMemcpy");
-        if (n_out) hip_check(hipMemcpy(d_demod.p, dem.data(), sizeof(float2) * n_out, hipMemcpyHostToDevice), "hipMemcpy");
-        hip_check(hipMemcpy(d_tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    }
-
-    // Per spline dataset and line: knot interval and the p+1 float basis values
-    // (GpuAlgorithm.cpp:657-692 does this per line with a constant-memory upload).
-    void upload_spline_basis() {
-        const int L = (int)lines.size();
-        for (auto& sp : splines) {
-            std::vector<SplineLineBasis> tab(std::max(L, 1));
-            for (int k = 0; k < L; k++) {
-                const float t = lines[k].t;
-                const int mu = knot_interval(sp->knots, t);
-                const int lo = mu - sp->degree, hi = mu;
-                if (lo < 0 || hi >= sp->num_cp)
-                    fail(BCSIM_ERR_INVALID_ARGUMENT, "timestamp outside the span of the spline basis");
-                SplineLineBasis& B = tab[k];
-                std::memset(&B, 0, sizeof(B));
-                B.lo = lo;
-                B.count = hi - lo + 1;
-                for (int i = 0; i < B.count; i++) B.b[i] = bspline_basis(lo + i, sp->degree, t, sp->knots);
-            }
-            sp->basis.ensure(sizeof(SplineLineBasis) * tab.size());
-            hip_check(hipMemcpy(sp->basis.p, tab.data(), sizeof(SplineLineBasis) * tab.size(),
-                                hipMemcpyHostToDevice), "hipMemcpy");
-        }
-    }
-
-    void prepare() {
-        require_configured();
-        if (lines.empty()) fail(BCSIM_ERR_INVALID_STATE, "No scanlines in scansequence");
-        if (profile < 0) fail(BCSIM_ERR_INVALID_STATE, "No beam profile is configured");
-        use_device();
-        if (dirty_scan || dirty_conv) {
-            update_shape();
-            upload_conv_tables();
-            d_lines.ensure(sizeof(LineGeom) * lines.size());
-            hip_check(hipMemcpy(d_lines.p, lines.data(), sizeof(LineGeom) * lines.size(), hipMemcpyHostToDevice),
-                      "hipMemcpy");
-            dirty_scan = dirty_conv = false;
-        }
-        if (dirty_basis) {
-            upload_spline_basis();
-            dirty_basis = false;
-        }
-        const size_t L = lines.size();
-        d_bins.ensure(L * (size_t)num_rf * (phase_delay ? 8 : 4));
-    }
-
-    ProjCommon proj_common(bool spline_rule) const {
-        ProjCommon c{};
-        c.lines = d_lines.as<LineGeom>();
-        c.num_lines = (int)lines.size();
-        c.num_rf = num_rf;
-        if (spline_rule) {
-            // r*2.0/(c*float(1/fs)) (CpuAlgorithm.cpp:161-162)
-            const float dt = (float)(1.0 / (double)fs);
-            c.bin_scale = 2.0;
-            c.bin_denom = (double)(sound_speed * dt);
-        } else {
-            // r*2.0*fs/c (CpuAlgorithm.cpp:75)
-            c.bin_scale = 2.0 * (double)fs;
-            c.bin_denom = (double)sound_speed;
-        }
-        c.bin_inv_denom = 1.0 / c.bin_denom;
-        c.phase_per_sample = (float)(2.0 * M_PI * (double)f_demod / (double)fs);
-        ProfileParams& P = c.prof;
-        if (profile == kProfileGaussian) {
-            P.inv_tsl = (float)(1.0 / (2.0 * (double)sigma_l * (double)sigma_l));
-            P.inv_tse = (float)(1.0 / (2.0 * (double)sigma_e * (double)sigma_e));
-        } else {
-            P.quad = lut.as<float4>();
-            P.nr = lut_nr;
-            P.nl = lut_nl;
-            P.ne = lut_ne;
-            P.r_min = lut_r0;
-            P.l_min = lut_l0;
-            P.e_min = lut_e0;
-            // grid spacing as LUTBeamProfile stores it: a float quotient (BeamProfile.cpp:68-70)
-            P.inv_dr = (float)(1.0 / (double)((lut_r1 - lut_r0) / (float)(lut_nr - 1)));
-            P.inv_dl = (float)(1.0 / (double)((lut_l1 - lut_l0) / (float)(lut_nl - 1)));
-            P.inv_de = (float)(1.0 / (double)((lut_e1 - lut_e0) / (float)(lut_ne - 1)));
-        }
-        return c;
-    }
-
-    // Launch shape of the projection: a tile of lines whose bins fit in LDS, and
-    // enough scatterer slices to give the 256 CUs a few workgroups each.
-    ProjLaunch proj_launch(size_t n) const {
-        ProjLaunch cfg{};
-        const size_t per_line = (size_t)num_rf * (phase_delay ? 8 : 4);
-        int tile = (int)std::min<size_t>(8, (size_t)(kLdsBytes - 1024) / per_line);
-        tile = std::max(1, std::min(tile, (int)lines.size()));
-        cfg.tile_lines = tile;
-        const int tiles = ((int)lines.size() + tile - 1) / tile;
-        const int target = 2048;
-        int slices = std::max(1, (target + tiles - 1) / tiles);
-        const int max_slices = std::max<int>(1, (int)((n + 2047) / 2048));
-        cfg.slices = std::min(slices, max_slices);
-        cfg.block = 256;
-        return cfg;
-    }
-
-    // ---------------------------------------------------------------- frame
-    struct Timed {
-        bcsim_sim* s;
-        const char* key;
-        hipEvent_t a = nullptr, b = nullptr;
-        Timed(bcsim_sim* s_, const char* k) : s(s_), key(k) {
-            if (!s->store_details) return;
-            hip_check(hipEventCreate(&a), "hipEventCreate");
-            hip_check(hipEventCreate(&b), "hipEventCreate");
-            hip_check(hipEventRecord(a, s->stream), "hipEventRecord");
-        }
-        void stop() {
-            if (!a) return;
-            hip_check(hipEventRecord(b, s->stream), "hipEventRecord");
-            s->pending.push_back({key, a, b});
-            a = b = nullptr;
-        }
-        ~Timed() {
-            if (a) (void)hipEventDestroy(a);
-            if (b) (void)hipEventDestroy(b);
-        }
-    };
-    struct Pending {
-        std::string key;
-        hipEvent_t a, b;
-    };
-    std::vector<Pending> pending;
-
-    void collect_timings() {
-        for (auto& p : pending) {
-            float ms = 0.f;
-            if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
-                debug[p.key].push_back((double)ms);
-            (void)hipEventDestroy(p.a);
-            (void)hipEventDestroy(p.b);
-        }
-        pending.clear();
-    }
-
-    // Beam support for the chunk cull (see CullParams): the radial window that
-    // gives a bin in [0, num_rf), and the profile's non-zero region.
-    CullParams cull_params(const ProjCommon& c) const {
-        CullParams cp{};
-        const double per_sample = c.bin_denom / c.bin_scale;  // metres per bin
-        double r_lo = -0.5 * per_sample, r_hi = ((double)num_rf - 0.5) * per_sample;
-        cp.l_lo = cp.e_lo = -INFINITY;
-        cp.l_hi = cp.e_hi = INFINITY;
-        cp.g_a = cp.g_b = 0.0f;
-        cp.g_max = INFINITY;
-        if (profile == kProfileLut) {
-            const double dr = (double)((lut_r1 - lut_r0) / (float)(lut_nr - 1));
-            const double dl = (double)((lut_l1 - lut_l0) / (float)(lut_nl - 1));
-            const double de = (double)((lut_e1 - lut_e0) / (float)(lut_ne - 1));
-            r_lo = std::max(r_lo, (double)lut_r0 - std::fabs(dr));
-            r_hi = std::min(r_hi, (double)lut_r0 + (lut_nr - 1) * std::fabs(dr));
-            cp.l_lo = (float)((double)lut_l0 - std::fabs(dl));
-            cp.l_hi = (float)((double)lut_l0 + (lut_nl - 1) * std::fabs(dl));
-            cp.e_lo = (float)((double)lut_e0 - std::fabs(de));
-            cp.e_hi = (float)((double)lut_e0 + (lut_ne - 1) * std::fabs(de));
-            if (!(dr > 0 && dl > 0 && de > 0)) {  // degenerate table: no culling
-                r_lo = -INFINITY;
-                r_hi = INFINITY;
-                cp.l_lo = cp.e_lo = -INFINITY;
-                cp.l_hi = cp.e_hi = INFINITY;
-            }
-        } else {
-            // expf(-x) rounds to zero for x > 103.98 (below half the smallest denormal);
-            // 110 leaves a 6 % margin on the exponent
-            cp.g_a = c.prof.inv_tsl;
-            cp.g_b = c.prof.inv_tse;
-            cp.g_max = (std::isfinite(cp.g_a) && std::isfinite(cp.g_b)) ? 110.0f : INFINITY;
-        }
-        cp.r_per_bin = (float)per_sample;
-        cp.r_lo = (float)r_lo;
-        cp.r_hi = (float)r_hi;
-        return cp;
-    }
-
-    bool shared_origin() const {
-        for (const auto& q : lines)
-            if (std::memcmp(&q.ox, &lines[0].ox, 3 * sizeof(float)) != 0) return false;
-        return true;
-    }
-
-    // Launch shape of the tile kernel: as many lines per workgroup as fit in
-    // LDS (at most 8), and enough chunk slices for about four workgroups per CU.
-    TileLaunch tile_launch(int nchunks) const {
-        TileLaunch cfg{};
-        const int L = (int)lines.size();
-        const size_t bin_bytes = phase_delay ? 8 : 4;
-        const size_t avail = (size_t)kLdsBytes - 64 * 8 - 256;  // keep room for a 64-entry list
-        // radial windows: one by default (measured fastest on the workloads of the tests
-        // and bench); more if one line's bins do not fit in LDS
-        int K = tune.windows > 0 ? tune.windows : 1;
-        K = std::max(1, std::min(K, num_rf));
-        while ((size_t)((num_rf + K - 1) / K) * bin_bytes > avail) K++;
-        const int W = (num_rf + K - 1) / K;
-        K = (num_rf + W - 1) / W;
-        const int tmax = (int)std::max<size_t>(1, std::min<size_t>(32, avail / ((size_t)W * bin_bytes)));
-        int T;
-        if (tune.lines > 0) {
-            T = std::min(tune.lines, tmax);
-        } else {
-            // as many lines as fit, but keep at least ~1024 workgroups
-            T = std::min(tmax, 16);
-            const int want = std::max(1, (L * K) / 1024);
-            T = std::max(1, std::min(T, want));
-        }
-        T = std::max(1, std::min(T, L));
-        const int blocks = ((L + T - 1) / T) * K;
-        int slices = tune.slices > 0 ? tune.slices : std::max(1, (512 + blocks - 1) / blocks);
-        slices = std::max(1, std::min(slices, std::max(1, nchunks / 16)));
-        cfg.tile_lines = T;
-        cfg.windows = K;
-        cfg.window_bins = W;
-        cfg.chunks_per_slice = std::max(1, (nchunks + slices - 1) / slices);
-        cfg.slices = std::max(1, (nchunks + cfg.chunks_per_slice - 1) / cfg.chunks_per_slice);
-        cfg.block = tune.block > 0 ? tune.block : 1024;
-        cfg.atomic = cfg.slices > 1 ? 1 : 0;
-        // the rest of the LDS holds the chunk list
-        const size_t left = (size_t)kLdsBytes - 256 - (size_t)T * W * bin_bytes;
-        cfg.list_cap = (int)std::min<size_t>(kMaxListCap, (left / 8) / 64 * 64);
-        return cfg;
-    }
-
-    void enqueue_frame(float2* d_out) {
-        prepare();
-        if (store_details) debug.clear();
-        const bool cplx = phase_delay;
-        const size_t L = lines.size();
-        const size_t bin_floats = L * (size_t)num_rf * (cplx ? 2 : 1);
-        float* bins = d_bins.as<float>();
-        const bool lut_profile = profile == kProfileLut;
-        const bool shared = shared_origin();
-
-        // One launch per non-empty dataset.  The bins are zeroed first unless a
-        // single launch writes every bin with plain stores.
-        size_t nonempty = 0;
-        for (auto& f : fixed) nonempty += f->n > 0;
-        for (auto& sp : splines) nonempty += sp->n > 0;
-        bool need_zero = true;
-        if (nonempty == 1 && !direct_mode) {
-            int only = 0;
-            for (auto& f : fixed) if (f->n) only = f->nchunks;
-            for (auto& sp : splines) if (sp->n) only = sp->nchunks;
-            need_zero = tile_launch(only).slices > 1;
-        } else if (nonempty == 1 && direct_mode) {
-            size_t only = 0;
-            for (auto& f : fixed) if (f->n) only = f->n;
-            for (auto& sp : splines) if (sp->n) only = sp->n;
-            need_zero = proj_launch(only).slices > 1;
-        }
-        if (need_zero) {
-            Timed t(this, "kernel_memset_ms");
-            hip_check(launch_fill_zero(bins, bin_floats, stream), "fill kernel");
-            t.stop();
-        }
-        const bool force_atomic = nonempty > 1;
-        for (auto& f : fixed) {
-            if (f->n == 0) continue;
-            const ProjCommon c = proj_common(false);
-            const FixedSet fs{f->pts.as<float4>(), (int)f->n};
-            Timed t(this, "fixed_projection_kernel_ms");
-            if (direct_mode) {
-                ProjLaunch cfg = proj_launch(f->n);
-                cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
-                hip_check(launch_project_fixed(c, fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
-                          "fixed projection kernel");
-            } else {
-                TileLaunch cfg = tile_launch(f->nchunks);
-                cfg.atomic |= force_atomic ? 1 : 0;
-                hip_check(launch_project_tile(c, cull_params(c), &fs, nullptr, f->boxes.as<ChunkBox>(), f->nchunks,
-                                              bins, arc, phase_delay, lut_profile, shared, cfg, stream),
-                          "fixed projection kernel");
-            }
-            t.stop();
-        }
-        for (auto& sp : splines) {
-            if (sp->n == 0) continue;
-            const ProjCommon c = proj_common(true);
-            SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
-                         sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
-            if (all_equal) {
-                // evaluate once at line 0's time, then project as fixed points with the
-                // spline bin rule (GpuAlgorithm.cpp:181-187, GpuScatterers.cpp:48-123)
-                SplineLineBasis B0;
-                host_basis_line0(*sp, B0);
-                sp->rendered.ensure(sp->n * sizeof(float4));
-                {
-                    Timed t(this, "kernel_render_ms");
-                    hip_check(launch_render_spline(ss, B0, sp->rendered.as<float4>(), stream), "render kernel");
-                    t.stop();
-                }
-                const FixedSet fs{sp->rendered.as<float4>(), (int)sp->n};
-                Timed t(this, "spline_projection_kernel_ms");
-                if (direct_mode) {
-                    ProjLaunch cfg = proj_launch(sp->n);
-                    cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
-                    hip_check(launch_project_fixed(c, fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
-                              "spline projection kernel");
-                } else {
-                    TileLaunch cfg = tile_launch(sp->nchunks);
-                    cfg.atomic |= force_atomic ? 1 : 0;
-                    hip_check(launch_project_tile(c, cull_params(c), &fs, nullptr, sp->boxes.as<ChunkBox>(),
-                                                  sp->nchunks, bins, arc, phase_delay, lut_profile, shared, cfg,
-                                                  stream),
-                              "spline projection kernel");
-                }
-                t.stop();
-            } else {
-                Timed t(this, "spline_projection_kernel_ms");
-                if (direct_mode) {
-                    ProjLaunch cfg = proj_launch(sp->n);
-                    cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
-                    hip_check(launch_project_spline(c, ss, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
-                              "spline projection kernel");
-                } else {
-                    TileLaunch cfg = tile_launch(sp->nchunks);
-                    cfg.atomic |= force_atomic ? 1 : 0;
-                    hip_check(launch_project_tile(c, cull_params(c), nullptr, &ss, sp->boxes.as<ChunkBox>(),
-                                                  sp->nchunks, bins, arc, phase_delay, lut_profile, false, cfg,
-                                                  stream),
-                              "spline projection kernel");
-                }
-                t.stop();
-            }
-        }
-        ConvParams p{};
-        p.bins = bins;
-        p.complex_bins = cplx ? 1 : 0;
-        p.num_lines = (int)L;
-        p.num_rf = num_rf;
-        p.fft_len = fft_len;
-        p.log2_len = log2_len;
-        p.twiddle_bits = log2_len / 2;
-        p.twiddles = d_tw.as<float2>();
-        p.H = d_H.as<float2>();
-        p.demod = d_demod.as<float2>();
-        p.center = center;
-        p.decimation = decimation;
-        p.n_out = n_out;
-        p.iq = d_out;
-        p.noise_amplitude = noise_amplitude > 0.0f ? noise_amplitude : 0.0f;
-        p.noise_seed = noise_seed;
-        p.noise_counter = noise_counter++;
-        {
-            Timed t(this, "kernel_convolve_demodulate_ms");
-            hip_check(launch_convolve_demod(p, stream), "convolve/demodulate kernel");
-            t.stop();
-        }
-    }
-
-    void host_basis_line0(const SplineDataset& sp, SplineLineBasis& B) {
-        const float t = lines[0].t;
-        const int mu = knot_interval(sp.knots, t);
-        std::memset(&B, 0, sizeof(B));
-        B.lo = mu - sp.degree;
-        B.count = sp.degree + 1;
-        for (int i = 0; i < B.count; i++) B.b[i] = bspline_basis(B.lo + i, sp.degree, t, sp.knots);
-    }
-
-    void simulate_device(void* d_out, size_t capacity, int* nl, int* ns) {
-        prepare();
-        const size_t need = lines.size() * (size_t)n_out;
-        if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
-        if (nl) *nl = (int)lines.size();
-        if (ns) *ns = n_out;
-        enqueue_frame(static_cast<float2*>(d_out));
-        if (store_details) collect_timings();
-    }
-
-    void simulate_host(float* out, size_t capacity, int* nl, int* ns) {
-        prepare();
-        const size_t need = lines.size() * (size_t)n_out;
-        if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
-        d_iq.ensure(need * sizeof(float2));
-        enqueue_frame(d_iq.as<float2>());
-        {
-            Timed t(this, "kernel_memcpy_ms");
-            hip_check(hipMemcpyAsync(out, d_iq.p, need * sizeof(float2), hipMemcpyDeviceToHost, stream),
-                      "hipMemcpyAsync");
-            t.stop();
-        }
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        if (store_details) collect_timings();
-        if (nl) *nl = (int)lines.size();
-        if (ns) *ns = n_out;
-    }
-
-    void debug_projection(float* out, size_t capacity, int* nl, int* nrf) {
-        if (!d_bins.p || dirty_scan) fail(BCSIM_ERR_INVALID_STATE, "no simulated frame");
-        const size_t L = lines.size();
-        if (nl) *nl = (int)L;
-        if (nrf) *nrf = num_rf;
-        if (!out) return;  // shape query
-        if (capacity < L * (size_t)num_rf) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        if (phase_delay) {
-            hip_check(hipMemcpy(out, d_bins.p, L * num_rf * sizeof(float2), hipMemcpyDeviceToHost), "hipMemcpy");
-        } else {
-            std::vector<float> re(L * (size_t)num_rf);
-            hip_check(hipMemcpy(re.data(), d_bins.p, re.size() * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
-            for (size_t i = 0; i < re.size(); i++) {
-                out[2 * i] = re[i];
-                out[2 * i + 1] = 0.0f;
-            }
-        }
-        if (nl) *nl = (int)L;
-        if (nrf) *nrf = num_rf;
-    }
-};
 
 // -------------------------------------------------------------------------------------------
 // C-ABI
 
-namespace {
-template <typename F>
-bcsim_status guarded(F&& f) {
-    try {
-        f();
-        g_last_error.clear();
-        return BCSIM_OK;
-    } catch (const BcsimError& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return BCSIM_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return BCSIM_ERR_INTERNAL;
-    }
-}
-
-void need(bcsim_sim* s) {
-    if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null simulator handle");
-}
-}  // namespace
 
 extern "C" {
 
-BCSIM_API const char* bcsim_last_error(void) { return g_last_error.c_str(); }
+BCSIM_API const char* bcsim_last_error(void) { return bcsim_hip::last_error().c_str(); }
 
 BCSIM_API bcsim_status bcsim_create(const char* sim_type, bcsim_sim** out) {
     return guarded([&] {

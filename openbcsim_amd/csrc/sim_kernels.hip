@@ -285,16 +285,15 @@ __device__ __forceinline__ float2 normal_pair(uint32_t seed, uint32_t counter, u
     return make_float2(rad * co, rad * s);
 }
 
+// One line: bins -> FFT -> x H -> IFFT -> extract, demodulate, decimate.
 template <int NT, int NMAX, bool COMPLEX_BINS>
-__global__ __launch_bounds__(NT) void convolve_demod_kernel(ConvParams p) {
-    extern __shared__ float2 sbuf[];
+__device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2* sbuf) {
     const int N = p.fft_len;
     float2* buf = sbuf;
     float2* twc = sbuf + N;
     const int n_coarse = N >> p.twiddle_bits;
     const int n_fine = 1 << p.twiddle_bits;
     float2* twf = twc + n_coarse;
-    const int line = blockIdx.x;
     for (int i = threadIdx.x; i < n_coarse + n_fine; i += NT) twc[i] = p.twiddles[i];
     // bins of this line, zero padded to N
     if (COMPLEX_BINS) {
@@ -321,6 +320,67 @@ __global__ __launch_bounds__(NT) void convolve_demod_kernel(ConvParams p) {
     // delay compensation, demodulation, decimation
     float2* out = p.iq + (size_t)line * p.n_out;
     for (int j = threadIdx.x; j < p.n_out; j += NT) out[j] = cmul(buf[p.center + j * p.decimation], p.demod[j]);
+}
+
+template <int NT, int NMAX, bool COMPLEX_BINS>
+__global__ __launch_bounds__(NT) void convolve_demod_kernel(ConvParams p) {
+    extern __shared__ float2 sbuf[];
+    conv_line<NT, NMAX, COMPLEX_BINS>(p, blockIdx.x, sbuf);
+}
+
+// Persistent convolution over the lines of many jobs: items are (job, line)
+// pairs taken from a global counter, longest transforms first.
+__global__ __launch_bounds__(1024) void convolve_batch_kernel(const ConvParams* __restrict__ jobs,
+                                                              const int2* __restrict__ items, int n_items,
+                                                              int* __restrict__ counter) {
+    extern __shared__ float2 sbuf[];
+    __shared__ int item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = item_s;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int2 w = items[it];
+        const ConvParams p = jobs[w.x];
+        if (p.complex_bins) conv_line<1024, kMaxFftLen, true>(p, w.y, sbuf);
+        else conv_line<1024, kMaxFftLen, false>(p, w.y, sbuf);
+        __syncthreads();
+    }
+}
+
+size_t convolve_batch_lds_bytes() {
+    // the largest transform plus its two-level twiddle table (coarse and fine halves of log2 N)
+    return ((size_t)kMaxFftLen + 2 * 128) * sizeof(float2);
+}
+
+hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int n_items, int* counter,
+                                 int workgroups, hipStream_t stream) {
+    if (n_items == 0) return hipSuccess;
+    const size_t lds = convolve_batch_lds_bytes();
+    hipError_t e = hipFuncSetAttribute((const void*)convolve_batch_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(convolve_batch_kernel, dim3(workgroups), dim3(1024), lds, stream, jobs, items, n_items,
+                       counter);
+    return hipGetLastError();
+}
+
+// Zero many regions and reset work counters in one launch (graph-safe: no memset nodes).
+__global__ void fill_segments_kernel(const FillSeg* segs, int n_segs, int* counters, int n_counters) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < n_counters) counters[threadIdx.x] = 0;
+    for (int s = 0; s < n_segs; s++) {
+        float* p = segs[s].p;
+        const size_t n = segs[s].n;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+            p[i] = 0.0f;
+    }
+}
+
+hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, int n_counters,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(fill_segments_kernel, dim3(1024), dim3(256), 0, stream, segs, n_segs, counters, n_counters);
+    return hipGetLastError();
 }
 
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
