@@ -98,6 +98,58 @@ def run_gpu(wl, steps: int, warmup: int, tile=None):
     return L, S, ms_per_step, {k: float(np.mean(v)) for k, v in kern.items() if v}, host_ms
 
 
+def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
+    """The 64 uneven jobs (scatterer counts log-uniform in [10k, 2M], fixed and spline,
+    32-512 lines, FFT 2048-16384, analytic or LUT): one frame of every job as plain
+    per-call frames (one synchronous simulate per job, the reference's call pattern)
+    and as one batched frame (persistent launches, graph replay)."""
+    from openbcsim_amd import workloads as W
+    from openbcsim_amd.batch import Batch
+    from openbcsim_amd.hip_rt import DeviceBuffer, hip
+    from openbcsim_amd.pyrfsim import RfSimulator
+
+    jobs = W.batched_jobs(num_jobs, seed=3, scale=scale)
+    sims = []
+    for wl in jobs:
+        s = RfSimulator("gpu")
+        wl.apply(s)
+        sims.append(s)
+    b = Batch(sims)
+    total, layout = b.layout()
+    out = DeviceBuffer(total * 8)
+    lines = sum(l for (_, l, _) in layout)
+    scat = sum(wl.num_scatterers * wl.num_lines for wl in jobs)
+
+    def per_call():
+        for s, (o, l, n) in zip(sims, layout):
+            s.simulate_lines_device(out.address + 8 * o, l * n)
+            s.synchronize()
+
+    def batched():
+        b.simulate_device(out.address, total)
+        b.synchronize()
+
+    res = {}
+    for name, fn in (("per_call", per_call), ("batched", batched)):
+        fn()
+        fn()
+        hip().hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = dt
+    out.close()
+    b.close()
+    for s in sims:
+        s.close()
+    return {"jobs": num_jobs, "lines_per_frame": lines, "scatterer_evals_per_frame": scat,
+            "per_call_ms": round(res["per_call"] * 1e3, 3), "batched_ms": round(res["batched"] * 1e3, 3),
+            "per_call_lines_per_s": round(lines / res["per_call"], 1),
+            "batched_lines_per_s": round(lines / res["batched"], 1),
+            "speedup": round(res["per_call"] / res["batched"], 3)}
+
+
 def cpu_baseline(wl, target_s: float = 12.0):
     """Reference CPU algorithm on the first lines of the scan (all scatterers)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -141,6 +193,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="volume_lut")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched-vs-per-call leg")
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="scatterer-count scale of the 64 jobs")
     ap.add_argument("--tile", default=None, help="projection launch shape 'windows,lines,block' (default: automatic)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy kernel of "
@@ -195,6 +249,8 @@ def main():
         "pcie_inclusive_lines_per_s": round(L / (host_ms * 1e-3), 1),
         "roofline": roofline,
     }
+    if not args.no_batch:
+        line["batched_64_jobs"] = run_batched(64, args.batch_scale)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(line), flush=True)
