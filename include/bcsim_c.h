@@ -59,10 +59,15 @@ BCSIM_API bcsim_status bcsim_destroy(bcsim_sim* sim);
 /* IAlgorithm::set_parameter (LibBCSim.hpp:50; BaseAlgorithm.cpp:48-87,
  * GpuAlgorithm.cpp:68-107).  Keys: verbose, sound_speed, noise_amplitude,
  * use_arc_projection, radial_decimation, phase_delay, gpu_device,
- * cuda_streams, threads_per_block, store_kernel_details; and two keys of this
- * implementation: noise_seed (counter-based noise stream) and projection_mode
+ * cuda_streams, threads_per_block, store_kernel_details; and keys of this
+ * implementation: noise_seed (counter-based noise stream); projection_mode
  * ("tiled", the culled tile kernel, default; "direct", every line-scatterer
- * pair, for checking). */
+ * pair, for checking); autotune (on/off, default off: time candidate launch
+ * shapes once per shape class, process-wide, and use the fastest); fft_radix
+ * (4 or 8; 0 = tuned or default); tile_lines, tile_block, tile_slices,
+ * tile_windows (tile-kernel launch shape; 0 = tuned or heuristic).
+ * Read-only keys of bcsim_get_parameter: launch_plan (shapes of the last frame)
+ * and autotune_report (every tuned class with its trial timings). */
 BCSIM_API bcsim_status bcsim_set_parameter(bcsim_sim* sim, const char* key, const char* value);
 
 /* IAlgorithm::get_parameter (LibBCSim.hpp:53; GpuAlgorithm.cpp:761-773).
@@ -176,7 +181,10 @@ typedef struct bcsim_batch bcsim_batch;
 BCSIM_API bcsim_status bcsim_batch_create(bcsim_batch** out);
 BCSIM_API bcsim_status bcsim_batch_destroy(bcsim_batch* batch);
 /* Keys: use_graph (on/off, default on), store_kernel_details (on/off: records
- * batch_frame_ms with HIP events, waiting for each frame). */
+ * batch_frame_ms with HIP events, waiting for each frame), autotune (on/off,
+ * default on: each job's shape class is tuned before its first batched frame),
+ * tile_lines (lines per tile for every job, 0 = each job's plan),
+ * items_per_workgroup (projection work items per persistent workgroup, default 32). */
 BCSIM_API bcsim_status bcsim_batch_set_parameter(bcsim_batch* batch, const char* key, const char* value);
 BCSIM_API bcsim_status bcsim_batch_add(bcsim_batch* batch, bcsim_sim* sim, int* job_index);
 BCSIM_API bcsim_status bcsim_batch_clear(bcsim_batch* batch);

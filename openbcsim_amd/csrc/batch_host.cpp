@@ -51,6 +51,9 @@ struct bcsim_batch {
     };
     std::vector<Render> renders;
     int workgroups = 256;
+    bool autotune = true;          // tune each job's shape class before its first batched frame
+    int tile_lines = 0;            // lines per tile for every job; 0 = each job's plan
+    int items_per_workgroup = 32;  // projection items per persistent workgroup (balance vs. overhead)
 
     // graph of the last capture
     hipGraph_t graph = nullptr;
@@ -95,6 +98,24 @@ struct bcsim_batch {
         invalidate();
     }
 
+    // Chunk slices per projection job for the persistent grid: items of about
+    // equal estimated cost (chunks x lines, splines evaluated per line weighted
+    // x3), `items_per_workgroup` per workgroup, so the heaviest-first order
+    // balances without cutting light jobs into slices that only add atomics.
+    void replan_slices(std::vector<ProjJob>& pjobs, const std::vector<double>& cost) const {
+        double total = 0.0;
+        for (double c : cost) total += c;
+        const double per_item = total / ((double)workgroups * items_per_workgroup);
+        for (size_t k = 0; k < pjobs.size(); k++) {
+            ProjJob& J = pjobs[k];
+            const double want_items = std::max(1.0, cost[k] / std::max(per_item, 1.0));
+            int slices = (int)std::ceil(want_items / J.blocks_per_slice);
+            slices = std::max(1, std::min(slices, std::max(1, J.nchunks / 16)));
+            J.g.chunks_per_slice = std::max(1, (J.nchunks + slices - 1) / slices);
+            J.g.slices = std::max(1, (J.nchunks + J.g.chunks_per_slice - 1) / J.g.chunks_per_slice);
+        }
+    }
+
     // Build the device-side job and item tables from the simulators' current state.
     void build() {
         if (sims.empty()) fail(BCSIM_ERR_INVALID_STATE, "empty batch");
@@ -103,12 +124,16 @@ struct bcsim_batch {
         hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // frames in flight use the tables
         drop_graph();
         table_out = nullptr;
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        workgroups = std::max(1, prop.multiProcessorCount);
         plan.clear();
         renders.clear();
         std::vector<ProjJob> pjobs;
         std::vector<double> pcost;  // per-item cost estimate of each projection job
         std::vector<ConvParams> cjobs;
         std::vector<FillSeg> fills;
+        std::vector<int> job_first_target, job_targets;
         size_t offset = 0;
         proj_lds = 0;
         for (bcsim_sim* s : sims) {
@@ -119,58 +144,61 @@ struct bcsim_batch {
             const bool cplx = s->phase_delay;
             const bool lut = s->profile == kProfileLut;
             const bool shared = s->shared_origin();
-            size_t nonempty = 0;
-            for (auto& f : s->fixed) nonempty += f->n > 0;
-            for (auto& sp : s->splines) nonempty += sp->n > 0;
-            auto add_job = [&](const ProjCommon& c, const FixedSet* fs, const SplineSet* ss, const DevBuf& boxes,
-                               int nchunks, bool is_spline) {
+            const std::vector<bcsim_sim::ProjTarget> targets = s->proj_targets();
+            const bool tune_now = autotune || s->autotune;
+            if (tune_now) {
+                // tuning trials project this job's data: rendered sets must hold points
+                s->enqueue_renders();
+            }
+            for (const auto& t : targets) {
+                const bool is_spline = std::strcmp(t.kind, "spline") == 0;
                 ProjJob J{};
-                J.c = c;
-                J.cp = s->cull_params(c);
-                if (fs) J.fs = *fs;
-                if (ss) J.ss = *ss;
-                J.boxes = boxes.as<ChunkBox>();
+                J.c = t.c;
+                J.cp = s->cull_params(t.c);
+                J.fs = t.fs;
+                J.ss = t.ss;
+                J.boxes = t.boxes;
                 J.bins = s->d_bins.as<float>();
-                J.g = s->tile_launch(nchunks);
+                J.g = s->plan_tile(t, tune_now);
+                if (tile_lines > 0) {
+                    TileTune tt = s->tune;
+                    tt.lines = tile_lines;
+                    J.g = s->tile_launch(t.nchunks, tt);
+                }
                 J.g.block = 1024;
-                J.g.atomic = (J.g.slices > 1 || nonempty > 1) ? 1 : 0;
-                J.nchunks = nchunks;
+                J.nchunks = t.nchunks;
                 J.flags = (s->arc ? kJobArc : 0) | (cplx ? kJobPhase : 0) | (lut ? kJobLut : 0) |
                           (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0);
-                const int tiles = (c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
+                const int tiles = (t.c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
                 J.blocks_per_slice = tiles * J.g.windows;
-                proj_lds = std::max(proj_lds, tile_lds_bytes(J.g.window_bins, cplx, J.g.tile_lines, J.g.list_cap));
                 pjobs.push_back(J);
-                pcost.push_back((double)J.g.chunks_per_slice * J.g.tile_lines * J.g.window_bins);
-            };
-            for (auto& f : s->fixed) {
-                if (f->n == 0) continue;
-                const FixedSet fs{f->pts.as<float4>(), (int)f->n};
-                add_job(s->proj_common(false), &fs, nullptr, f->boxes, f->nchunks, false);
-            }
-            for (auto& sp : s->splines) {
-                if (sp->n == 0) continue;
-                const SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
-                                   sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
-                if (s->all_equal) {
-                    Render r{ss, {}, nullptr};
-                    s->host_basis_line0(*sp, r.basis);
-                    sp->rendered.ensure(sp->n * sizeof(float4));
-                    r.out = sp->rendered.as<float4>();
+                pcost.push_back((double)t.nchunks * t.c.num_lines * (is_spline ? 3.0 : 1.0));
+                if (std::strcmp(t.kind, "rendered") == 0) {
+                    Render r{t.ss, {}, t.sp->rendered.as<float4>()};
+                    s->host_basis_line0(*t.sp, r.basis);
                     renders.push_back(r);
-                    const FixedSet fs{r.out, (int)sp->n};
-                    add_job(s->proj_common(true), &fs, nullptr, sp->boxes, sp->nchunks, false);
-                } else {
-                    add_job(s->proj_common(true), nullptr, &ss, sp->boxes, sp->nchunks, true);
                 }
             }
-            // bins that are accumulated (or not written at all) start from zero
-            bool single_store = false;
-            if (nonempty == 1) single_store = pjobs.back().g.atomic == 0;
-            if (!single_store)
-                fills.push_back(FillSeg{s->d_bins.as<float>(), s->lines.size() * (size_t)s->num_rf * (cplx ? 2 : 1)});
+            job_first_target.push_back((int)pjobs.size() - (int)targets.size());
+            job_targets.push_back((int)targets.size());
             cjobs.push_back(s->conv_params(nullptr));  // iq pointer set per call (offset into the output)
+            cjobs.back().radix = s->plan_radix(tune_now);
+            s->quiesce();  // renders and trials on the job's own stream are done before the batch runs
         }
+        replan_slices(pjobs, pcost);
+        // bins that are accumulated (or not written at all) start from zero
+        for (size_t k = 0; k < sims.size(); k++) {
+            bcsim_sim* s = sims[k];
+            const int first = job_first_target[k], n = job_targets[k];
+            bool single_store = n == 1 && pjobs[first].g.slices == 1;
+            for (int q = first; q < first + n; q++) pjobs[q].g.atomic = single_store ? 0 : 1;
+            if (!single_store)
+                fills.push_back(FillSeg{s->d_bins.as<float>(),
+                                        s->lines.size() * (size_t)s->num_rf * (s->phase_delay ? 2 : 1)});
+        }
+        for (const ProjJob& J : pjobs)
+            proj_lds = std::max(proj_lds, tile_lds_bytes(J.g.window_bins, (J.flags & kJobPhase) != 0, J.g.tile_lines,
+                                                         J.g.list_cap));
         total_out = offset;
 
         // items, heaviest first
@@ -204,9 +232,6 @@ struct bcsim_batch {
         n_proj_items = (int)pitems.size();
         n_conv_items = (int)citems.size();
         n_fill = (int)fills.size();
-        hipDeviceProp_t prop;
-        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-        workgroups = std::max(1, prop.multiProcessorCount);
         planned = true;
     }
     std::vector<ConvParams> conv_host;
@@ -301,6 +326,19 @@ BCSIM_API bcsim_status bcsim_batch_set_parameter(bcsim_batch* b, const char* key
             b->drop_graph();
         } else if (k == "store_kernel_details") {
             b->store_details = parse_bool(value);
+        } else if (k == "autotune") {
+            b->autotune = parse_bool(value);
+            b->invalidate();
+        } else if (k == "tile_lines") {
+            const int v = parse_int(value);
+            if (v < 0 || v > 32) fail(BCSIM_ERR_INVALID_ARGUMENT, "tile_lines must be in [0, 32]");
+            b->tile_lines = v;
+            b->invalidate();
+        } else if (k == "items_per_workgroup") {
+            const int v = parse_int(value);
+            if (v < 1 || v > 1024) fail(BCSIM_ERR_INVALID_ARGUMENT, "items_per_workgroup must be in [1, 1024]");
+            b->items_per_workgroup = v;
+            b->invalidate();
         } else {
             fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal parameter name: '" + k + "'");
         }

@@ -24,6 +24,7 @@ namespace bcsim_hip {
 
 constexpr int kMaxSplineDegree = 4;   // common_definitions.h:2
 constexpr int kMaxFftLen = 16384;     // one line in 128 KiB of LDS
+constexpr int kDefaultRadix = 8;       // FFT pass radix when not set or tuned (8 measured faster at every length)
 constexpr int kMinFftLen = 16;
 constexpr int kLdsBytes = 160 * 1024; // gfx950 LDS per workgroup
 
@@ -109,6 +110,7 @@ struct ConvParams {
     float noise_amplitude;
     uint32_t noise_seed;
     uint32_t noise_counter;
+    int radix;              // largest FFT pass radix: 4 or 8
 };
 
 // ---- culled tile projection (project_tile.hip) ----

@@ -199,6 +199,44 @@ struct Twiddle {
     }
 };
 
+// 4-point DFT (forward: exp(-2 pi i nk / 4); inverse: conjugate kernel).
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 u0, float2 u1, float2 u2, float2 u3, float2& y0, float2& y1,
+                                     float2& y2, float2& y3) {
+    const float2 a0 = cadd(u0, u2);
+    const float2 a1 = csub(u0, u2);
+    const float2 a2 = cadd(u1, u3);
+    const float2 a3 = rot90<INV>(csub(u1, u3));
+    y0 = cadd(a0, a2);
+    y2 = csub(a0, a2);
+    y1 = cadd(a1, a3);
+    y3 = csub(a1, a3);
+}
+
+// 8-point DFT as two 4-point DFTs of the even and odd inputs joined by the
+// eighth roots of unity: X[k] = E[k] + w^k O[k], X[k+4] = E[k] - w^k O[k].
+template <bool INV>
+__device__ __forceinline__ void dft8(const float2* x, float2* y) {
+    constexpr float h = 0.70710678118654752f;  // 1/sqrt(2)
+    float2 e[4], o[4];
+    dft4<INV>(x[0], x[2], x[4], x[6], e[0], e[1], e[2], e[3]);
+    dft4<INV>(x[1], x[3], x[5], x[7], o[0], o[1], o[2], o[3]);
+    // w = exp(-i pi/4) forward, exp(+i pi/4) inverse
+    const float2 o1 = INV ? make_float2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                          : make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    const float2 o2 = rot90<INV>(o[2]);
+    const float2 o3 = INV ? make_float2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                          : make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+    y[0] = cadd(e[0], o[0]);
+    y[4] = csub(e[0], o[0]);
+    y[1] = cadd(e[1], o1);
+    y[5] = csub(e[1], o1);
+    y[2] = cadd(e[2], o2);
+    y[6] = csub(e[2], o2);
+    y[3] = cadd(e[3], o3);
+    y[7] = csub(e[3], o3);
+}
+
 // One Stockham autosort pass of radix R over N points held in LDS.  Ls is the
 // length of the sub-transforms already formed.  Every thread first reads all
 // its inputs, the workgroup synchronises, then every thread writes its outputs,
@@ -230,15 +268,10 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int N, int Ls, const 
             if (R == 2) {
                 y[0] = cadd(x[it][0], x[it][1]);
                 y[1] = csub(x[it][0], x[it][1]);
+            } else if (R == 4) {
+                dft4<INV>(x[it][0], x[it][1], x[it][2], x[it][3], y[0], y[1], y[2], y[3]);
             } else {
-                const float2 a0 = cadd(x[it][0], x[it][2]);
-                const float2 a1 = csub(x[it][0], x[it][2]);
-                const float2 a2 = cadd(x[it][1], x[it][3]);
-                const float2 a3 = rot90<INV>(csub(x[it][1], x[it][3]));
-                y[0] = cadd(a0, a2);
-                y[2] = csub(a0, a2);
-                y[1] = cadd(a1, a3);
-                y[3] = csub(a1, a3);
+                dft8<INV>(x[it], y);
             }
             const int base = (j - k) * R + k;
 #pragma unroll
@@ -248,16 +281,33 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int N, int Ls, const 
     __syncthreads();
 }
 
-template <int NT, int NMAX, bool INV>
+// Power-of-two FFT in LDS: radix-RMAX passes (4 or 8) after one smaller pass
+// that takes up the remainder of log2 N.
+template <int NT, int NMAX, bool INV, int RMAX>
 __device__ __forceinline__ void fft_lds(float2* buf, int N, int log2n, const Twiddle& tw) {
     int Ls = 1;
-    if (log2n & 1) {
-        stockham_pass<2, NT, NMAX, INV>(buf, N, Ls, tw);
-        Ls = 2;
-    }
-    while (Ls < N) {
-        stockham_pass<4, NT, NMAX, INV>(buf, N, Ls, tw);
-        Ls *= 4;
+    if (RMAX == 8) {
+        const int rem = log2n % 3;
+        if (rem == 1) {
+            stockham_pass<2, NT, NMAX, INV>(buf, N, Ls, tw);
+            Ls = 2;
+        } else if (rem == 2) {
+            stockham_pass<4, NT, NMAX, INV>(buf, N, Ls, tw);
+            Ls = 4;
+        }
+        while (Ls < N) {
+            stockham_pass<8, NT, NMAX, INV>(buf, N, Ls, tw);
+            Ls *= 8;
+        }
+    } else {
+        if (log2n & 1) {
+            stockham_pass<2, NT, NMAX, INV>(buf, N, Ls, tw);
+            Ls = 2;
+        }
+        while (Ls < N) {
+            stockham_pass<4, NT, NMAX, INV>(buf, N, Ls, tw);
+            Ls *= 4;
+        }
     }
 }
 
@@ -286,7 +336,7 @@ __device__ __forceinline__ float2 normal_pair(uint32_t seed, uint32_t counter, u
 }
 
 // One line: bins -> FFT -> x H -> IFFT -> extract, demodulate, decimate.
-template <int NT, int NMAX, bool COMPLEX_BINS>
+template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX>
 __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2* sbuf) {
     const int N = p.fft_len;
     float2* buf = sbuf;
@@ -311,21 +361,21 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
     }
     __syncthreads();
     Twiddle tw{twc, twf, p.twiddle_bits, n_fine - 1};
-    fft_lds<NT, NMAX, false>(buf, N, p.log2_len, tw);
+    fft_lds<NT, NMAX, false, RMAX>(buf, N, p.log2_len, tw);
     // excitation spectrum with Hilbert mask and 1/N; zero above N/2 (the mask)
     const int half = N / 2;
     for (int k = threadIdx.x; k < N; k += NT) buf[k] = k <= half ? cmul(buf[k], p.H[k]) : make_float2(0.f, 0.f);
     __syncthreads();
-    fft_lds<NT, NMAX, true>(buf, N, p.log2_len, tw);
+    fft_lds<NT, NMAX, true, RMAX>(buf, N, p.log2_len, tw);
     // delay compensation, demodulation, decimation
     float2* out = p.iq + (size_t)line * p.n_out;
     for (int j = threadIdx.x; j < p.n_out; j += NT) out[j] = cmul(buf[p.center + j * p.decimation], p.demod[j]);
 }
 
-template <int NT, int NMAX, bool COMPLEX_BINS>
+template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX>
 __global__ __launch_bounds__(NT) void convolve_demod_kernel(ConvParams p) {
     extern __shared__ float2 sbuf[];
-    conv_line<NT, NMAX, COMPLEX_BINS>(p, blockIdx.x, sbuf);
+    conv_line<NT, NMAX, COMPLEX_BINS, RMAX>(p, blockIdx.x, sbuf);
 }
 
 // Persistent convolution over the lines of many jobs: items are (job, line)
@@ -343,8 +393,13 @@ __global__ __launch_bounds__(1024) void convolve_batch_kernel(const ConvParams* 
         if (it >= n_items) break;
         const int2 w = items[it];
         const ConvParams p = jobs[w.x];
-        if (p.complex_bins) conv_line<1024, kMaxFftLen, true>(p, w.y, sbuf);
-        else conv_line<1024, kMaxFftLen, false>(p, w.y, sbuf);
+        if (p.radix == 8) {
+            if (p.complex_bins) conv_line<1024, kMaxFftLen, true, 8>(p, w.y, sbuf);
+            else conv_line<1024, kMaxFftLen, false, 8>(p, w.y, sbuf);
+        } else {
+            if (p.complex_bins) conv_line<1024, kMaxFftLen, true, 4>(p, w.y, sbuf);
+            else conv_line<1024, kMaxFftLen, false, 4>(p, w.y, sbuf);
+        }
         __syncthreads();
     }
 }
@@ -385,32 +440,28 @@ hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, 
 
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
 
+template <int NT, int NMAX, bool CB, int R>
+static hipError_t launch_conv(const ConvParams& p, size_t lds, hipStream_t stream) {
+    const void* k = (const void*)convolve_demod_kernel<NT, NMAX, CB, R>;
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((convolve_demod_kernel<NT, NMAX, CB, R>), dim3(p.num_lines), dim3(NT), lds, stream, p);
+    return hipGetLastError();
+}
+
 hipError_t launch_convolve_demod(const ConvParams& p, hipStream_t stream) {
     if (p.num_lines == 0) return hipSuccess;
     const int nt = conv_block_threads(p.fft_len);
     const size_t lds = ((size_t)p.fft_len + (p.fft_len >> p.twiddle_bits) + (1u << p.twiddle_bits)) * sizeof(float2);
-    const void* k;
+    const bool cb = p.complex_bins != 0, r8 = p.radix == 8;
     if (nt == 1024) {
-        k = p.complex_bins ? (const void*)convolve_demod_kernel<1024, kMaxFftLen, true>
-                           : (const void*)convolve_demod_kernel<1024, kMaxFftLen, false>;
-    } else {
-        k = p.complex_bins ? (const void*)convolve_demod_kernel<256, 4096, true>
-                           : (const void*)convolve_demod_kernel<256, 4096, false>;
+        if (r8) return cb ? launch_conv<1024, kMaxFftLen, true, 8>(p, lds, stream)
+                          : launch_conv<1024, kMaxFftLen, false, 8>(p, lds, stream);
+        return cb ? launch_conv<1024, kMaxFftLen, true, 4>(p, lds, stream)
+                  : launch_conv<1024, kMaxFftLen, false, 4>(p, lds, stream);
     }
-    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    if (nt == 1024) {
-        if (p.complex_bins)
-            hipLaunchKernelGGL((convolve_demod_kernel<1024, kMaxFftLen, true>), dim3(p.num_lines), dim3(nt), lds, stream, p);
-        else
-            hipLaunchKernelGGL((convolve_demod_kernel<1024, kMaxFftLen, false>), dim3(p.num_lines), dim3(nt), lds, stream, p);
-    } else {
-        if (p.complex_bins)
-            hipLaunchKernelGGL((convolve_demod_kernel<256, 4096, true>), dim3(p.num_lines), dim3(nt), lds, stream, p);
-        else
-            hipLaunchKernelGGL((convolve_demod_kernel<256, 4096, false>), dim3(p.num_lines), dim3(nt), lds, stream, p);
-    }
-    return hipGetLastError();
+    if (r8) return cb ? launch_conv<256, 4096, true, 8>(p, lds, stream) : launch_conv<256, 4096, false, 8>(p, lds, stream);
+    return cb ? launch_conv<256, 4096, true, 4>(p, lds, stream) : launch_conv<256, 4096, false, 4>(p, lds, stream);
 }
 
 }  // namespace bcsim_hip

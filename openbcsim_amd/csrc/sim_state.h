@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/bcsim_c.h"
+#include "autotune.h"
 #include "sim_device.h"
 
 
@@ -237,10 +238,11 @@ struct bcsim_sim {
     bool store_details = false;
     uint64_t version = 0;      // bumped by every setter; batches rebuild their plans on change
     bool direct_mode = false;  // projection_mode: "tiled" (culled tile kernel) or "direct"
-    // launch shape of the tile kernel; 0 = chosen from the frame shape
-    struct {
-        int windows = 0, lines = 0, block = 0, slices = 0;
-    } tune;
+    // launch shape of the tile kernel set by the user; 0 = tuned or heuristic
+    TileTune tune;
+    bool autotune = false;  // time candidate launch shapes once per shape class
+    int radix = 0;          // FFT pass radix set by the user (4 or 8); 0 = tuned or default
+    std::string last_plan;  // launch shapes of the last frame, for get_parameter("launch_plan")
 
     hipStream_t stream = nullptr;
     bcsim_log_fn log_fn = nullptr;
@@ -346,6 +348,12 @@ struct bcsim_sim {
             if (value == "tiled") direct_mode = false;
             else if (value == "direct") direct_mode = true;
             else fail(BCSIM_ERR_INVALID_ARGUMENT, "projection_mode must be 'tiled' or 'direct'");
+        } else if (key == "autotune") {
+            autotune = parse_bool(value);
+        } else if (key == "fft_radix") {
+            const int v = parse_int(value);
+            if (v != 0 && v != 4 && v != 8) fail(BCSIM_ERR_INVALID_ARGUMENT, "fft_radix must be 4, 8 or 0 (automatic)");
+            radix = v;
         } else if (key == "tile_windows" || key == "tile_lines" || key == "tile_block" || key == "tile_slices") {
             const int v = parse_int(value);
             if (v < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, key + " must be >= 0 (0 = automatic)");
@@ -384,6 +392,10 @@ struct bcsim_sim {
         if (key == "threads_per_block") return std::to_string(threads_per_block);
         if (key == "store_kernel_details") return store_details ? "on" : "off";
         if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
+        if (key == "autotune") return autotune ? "on" : "off";
+        if (key == "fft_radix") return std::to_string(radix);
+        if (key == "launch_plan") return last_plan;
+        if (key == "autotune_report") return TuneCache::get().report();
         if (key == "tile_windows") return std::to_string(tune.windows);
         if (key == "tile_lines") return std::to_string(tune.lines);
         if (key == "tile_block") return std::to_string(tune.block);
@@ -872,9 +884,11 @@ struct bcsim_sim {
         return true;
     }
 
-    // Launch shape of the tile kernel: as many lines per workgroup as fit in
-    // LDS (at most 8), and enough chunk slices for about four workgroups per CU.
-    TileLaunch tile_launch(int nchunks) const {
+    // Launch shape of the tile kernel for `tune` (0 fields: heuristic): one
+    // radial window unless a line's bins overflow LDS, up to 16 lines per
+    // workgroup (while at least ~1024 workgroups remain), and enough chunk
+    // slices for about 512 workgroups.
+    TileLaunch tile_launch(int nchunks, const TileTune& tune) const {
         TileLaunch cfg{};
         const int L = (int)lines.size();
         const size_t bin_bytes = phase_delay ? 8 : 4;
@@ -913,6 +927,171 @@ struct bcsim_sim {
         return cfg;
     }
 
+    // ---------------------------------------------------------------- launch planning
+    // One projection launch of the frame: a dataset and how it is projected.
+    struct ProjTarget {
+        const char* kind;  // "fixed", "rendered" (spline at one time) or "spline"
+        ProjCommon c;
+        FixedSet fs{};
+        SplineSet ss{};
+        const ChunkBox* boxes = nullptr;
+        int nchunks = 0;
+        size_t n = 0;
+        SplineDataset* sp = nullptr;  // spline kinds
+    };
+
+    std::vector<ProjTarget> proj_targets() {
+        std::vector<ProjTarget> out;
+        for (auto& f : fixed) {
+            if (f->n == 0) continue;
+            ProjTarget t{"fixed", proj_common(false)};
+            t.fs = FixedSet{f->pts.as<float4>(), (int)f->n};
+            t.boxes = f->boxes.as<ChunkBox>();
+            t.nchunks = f->nchunks;
+            t.n = f->n;
+            out.push_back(t);
+        }
+        for (auto& sp : splines) {
+            if (sp->n == 0) continue;
+            ProjTarget t{all_equal ? "rendered" : "spline", proj_common(true)};
+            t.ss = SplineSet{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
+                             sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
+            if (all_equal) {
+                sp->rendered.ensure(sp->n * sizeof(float4));
+                t.fs = FixedSet{sp->rendered.as<float4>(), (int)sp->n};
+            }
+            t.boxes = sp->boxes.as<ChunkBox>();
+            t.nchunks = sp->nchunks;
+            t.n = sp->n;
+            t.sp = sp.get();
+            out.push_back(t);
+        }
+        return out;
+    }
+
+    // Render the all-timestamps-equal spline sets at line 0's time
+    // (GpuAlgorithm.cpp:181-187, GpuScatterers.cpp:48-123).
+    void enqueue_renders() {
+        if (!all_equal) return;
+        for (auto& sp : splines) {
+            if (sp->n == 0) continue;
+            const SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
+                               sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
+            SplineLineBasis B0;
+            host_basis_line0(*sp, B0);
+            sp->rendered.ensure(sp->n * sizeof(float4));
+            Timed t(this, "kernel_render_ms");
+            hip_check(launch_render_spline(ss, B0, sp->rendered.as<float4>(), stream), "render kernel");
+            t.stop();
+        }
+    }
+
+    void launch_target(const ProjTarget& t, const TileLaunch& cfg, float* bins) {
+        const bool is_spline = std::strcmp(t.kind, "spline") == 0;
+        hip_check(launch_project_tile(t.c, cull_params(t.c), is_spline ? nullptr : &t.fs, is_spline ? &t.ss : nullptr,
+                                      t.boxes, t.nchunks, bins, arc, phase_delay, profile == kProfileLut,
+                                      shared_origin() && !is_spline, cfg, stream),
+                  is_spline ? "spline projection kernel" : "fixed projection kernel");
+    }
+
+    std::string shape_class(const char* kind, int nchunks) const {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "tile/%s/%s/%s/%s/L2^%d/C2^%d/F%d", kind,
+                      profile == kProfileLut ? "lut" : "gauss", phase_delay ? "complex" : "real",
+                      arc ? "arc" : "noarc", ceil_log2((long long)lines.size()), ceil_log2(nchunks), fft_len);
+        return buf;
+    }
+
+    // Tuned tile shape of this target's class: looked up, or found now by a
+    // coordinate search from the heuristic shape (lines per tile, then total
+    // workgroups, then block size), each candidate timed on this frame's data.
+    // Writes the bins: call before they are cleared for the frame.
+    TileTune tuned_tile(const ProjTarget& t) {
+        const std::string key = shape_class(t.kind, t.nchunks);
+        TileTune best;
+        if (TuneCache::get().find_tile(key, best)) return best;
+        float* bins = d_bins.as<float>();
+        std::string trials;
+        auto cost = [&](const TileTune& tt) {
+            const TileLaunch cfg = tile_launch(t.nchunks, tt);
+            const double ms = time_launches(stream, [&] { launch_target(t, cfg, bins); }, 2);
+            char b[48];
+            std::snprintf(b, sizeof b, " [%d,%d,%d,%d]=%.3f", cfg.tile_lines, cfg.block, cfg.slices, cfg.windows, ms);
+            trials += b;
+            return ms;
+        };
+        double best_ms = cost(best);
+        const int tmax = tile_launch(t.nchunks, TileTune{0, 32, 0, 0}).tile_lines;
+        for (int T = 1; T <= tmax; T *= 2) {
+            TileTune c = best;
+            c.lines = T;
+            const double ms = cost(c);
+            if (ms < best_ms) best = c, best_ms = ms;
+        }
+        const TileLaunch cur = tile_launch(t.nchunks, best);
+        const int blocks = (((int)lines.size() + cur.tile_lines - 1) / cur.tile_lines) * cur.windows;
+        for (int wgs : {256, 1024, 2048, 4096}) {
+            TileTune c = best;
+            c.lines = cur.tile_lines;
+            c.slices = std::max(1, (wgs + blocks - 1) / blocks);
+            if (c.slices == cur.slices) continue;
+            const double ms = cost(c);
+            if (ms < best_ms) best = c, best_ms = ms;
+        }
+        for (int b : {256, 512}) {
+            TileTune c = best;
+            c.block = b;
+            const double ms = cost(c);
+            if (ms < best_ms) best = c, best_ms = ms;
+        }
+        char head[64];
+        std::snprintf(head, sizeof head, "%.3f ms with ", best_ms);
+        TuneCache::get().put_tile(key, best, head + describe(best) + "; trials [lines,block,slices,windows]:" + trials);
+        return best;
+    }
+
+    // Launch shape for a target: shape fields set by the user win, then the
+    // tuned shape (when tuning is on), then the heuristic.
+    TileLaunch plan_tile(const ProjTarget& t, bool tune_now) {
+        TileTune eff = tune;
+        if (tune_now) {
+            const TileTune tt = tuned_tile(t);
+            if (!eff.windows) eff.windows = tt.windows;
+            if (!eff.lines) eff.lines = tt.lines;
+            if (!eff.block) eff.block = tt.block;
+            if (!eff.slices) eff.slices = tt.slices;
+        }
+        return tile_launch(t.nchunks, eff);
+    }
+
+    // FFT pass radix of this frame's convolution: set by the user, tuned per
+    // (FFT length, bin type, line count class), or the default.
+    int plan_radix(bool tune_now) {
+        if (radix) return radix;
+        if (!tune_now || lines.empty()) return kDefaultRadix;
+        char key[96];
+        std::snprintf(key, sizeof key, "conv/F%d/%s/L2^%d", fft_len, phase_delay ? "complex" : "real",
+                      ceil_log2((long long)lines.size()));
+        int r = 0;
+        if (TuneCache::get().find_radix(key, r)) return r;
+        d_iq.ensure(lines.size() * (size_t)n_out * sizeof(float2));
+        const uint32_t counter = noise_counter;  // trials do not advance the noise stream
+        ConvParams p = conv_params(d_iq.as<float2>());
+        noise_counter = counter;
+        std::string trials;
+        double best_ms = 0.0;
+        for (int cand : {4, 8}) {
+            p.radix = cand;
+            const double ms = time_launches(stream, [&] { hip_check(launch_convolve_demod(p, stream), "convolve"); }, 2);
+            char b[32];
+            std::snprintf(b, sizeof b, " r%d=%.3f", cand, ms);
+            trials += b;
+            if (r == 0 || ms < best_ms) r = cand, best_ms = ms;
+        }
+        TuneCache::get().put_radix(key, r, "radix " + std::to_string(r) + ";" + trials);
+        return r;
+    }
+
     void enqueue_frame(float2* d_out) {
         prepare();
         if (store_details) debug.clear();
@@ -921,101 +1100,57 @@ struct bcsim_sim {
         const size_t bin_floats = L * (size_t)num_rf * (cplx ? 2 : 1);
         float* bins = d_bins.as<float>();
         const bool lut_profile = profile == kProfileLut;
-        const bool shared = shared_origin();
 
-        // One launch per non-empty dataset.  The bins are zeroed first unless a
-        // single launch writes every bin with plain stores.
-        size_t nonempty = 0;
-        for (auto& f : fixed) nonempty += f->n > 0;
-        for (auto& sp : splines) nonempty += sp->n > 0;
-        bool need_zero = true;
-        if (nonempty == 1 && !direct_mode) {
-            int only = 0;
-            for (auto& f : fixed) if (f->n) only = f->nchunks;
-            for (auto& sp : splines) if (sp->n) only = sp->nchunks;
-            need_zero = tile_launch(only).slices > 1;
-        } else if (nonempty == 1 && direct_mode) {
-            size_t only = 0;
-            for (auto& f : fixed) if (f->n) only = f->n;
-            for (auto& sp : splines) if (sp->n) only = sp->n;
-            need_zero = proj_launch(only).slices > 1;
+        enqueue_renders();
+        const std::vector<ProjTarget> targets = proj_targets();
+        // plan every launch first: tuning trials write the bins
+        std::vector<TileLaunch> plans;
+        std::vector<ProjLaunch> direct_plans;
+        last_plan.clear();
+        for (const ProjTarget& t : targets) {
+            if (direct_mode) {
+                direct_plans.push_back(proj_launch(t.n));
+            } else {
+                plans.push_back(plan_tile(t, autotune));
+                const TileLaunch& g = plans.back();
+                last_plan += std::string(t.kind) + ": lines=" + std::to_string(g.tile_lines) +
+                             " block=" + std::to_string(g.block) + " slices=" + std::to_string(g.slices) +
+                             " windows=" + std::to_string(g.windows) + "\n";
+            }
         }
+        const int frame_radix = plan_radix(autotune);
+        last_plan += "convolve: radix=" + std::to_string(frame_radix) + "\n";
+
+        // The bins are zeroed first unless a single launch writes every bin with plain stores.
+        const bool single = targets.size() == 1;
+        const bool need_zero = !single || (direct_mode ? direct_plans[0].slices > 1 : plans[0].slices > 1);
         if (need_zero) {
             Timed t(this, "kernel_memset_ms");
             hip_check(launch_fill_zero(bins, bin_floats, stream), "fill kernel");
             t.stop();
         }
-        const bool force_atomic = nonempty > 1;
-        for (auto& f : fixed) {
-            if (f->n == 0) continue;
-            const ProjCommon c = proj_common(false);
-            const FixedSet fs{f->pts.as<float4>(), (int)f->n};
-            Timed t(this, "fixed_projection_kernel_ms");
+        for (size_t i = 0; i < targets.size(); i++) {
+            const ProjTarget& t = targets[i];
+            const bool is_spline = std::strcmp(t.kind, "fixed") != 0;
+            Timed tm(this, is_spline ? "spline_projection_kernel_ms" : "fixed_projection_kernel_ms");
             if (direct_mode) {
-                ProjLaunch cfg = proj_launch(f->n);
-                cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
-                hip_check(launch_project_fixed(c, fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
-                          "fixed projection kernel");
+                ProjLaunch cfg = direct_plans[i];
+                cfg.atomic = (cfg.slices > 1 || !single) ? 1 : 0;
+                if (std::strcmp(t.kind, "spline") == 0)
+                    hip_check(launch_project_spline(t.c, t.ss, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
+                              "spline projection kernel");
+                else
+                    hip_check(launch_project_fixed(t.c, t.fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
+                              is_spline ? "spline projection kernel" : "fixed projection kernel");
             } else {
-                TileLaunch cfg = tile_launch(f->nchunks);
-                cfg.atomic |= force_atomic ? 1 : 0;
-                hip_check(launch_project_tile(c, cull_params(c), &fs, nullptr, f->boxes.as<ChunkBox>(), f->nchunks,
-                                              bins, arc, phase_delay, lut_profile, shared, cfg, stream),
-                          "fixed projection kernel");
+                TileLaunch cfg = plans[i];
+                cfg.atomic |= single ? 0 : 1;
+                launch_target(t, cfg, bins);
             }
-            t.stop();
+            tm.stop();
         }
-        for (auto& sp : splines) {
-            if (sp->n == 0) continue;
-            const ProjCommon c = proj_common(true);
-            SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
-                         sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
-            if (all_equal) {
-                // evaluate once at line 0's time, then project as fixed points with the
-                // spline bin rule (GpuAlgorithm.cpp:181-187, GpuScatterers.cpp:48-123)
-                SplineLineBasis B0;
-                host_basis_line0(*sp, B0);
-                sp->rendered.ensure(sp->n * sizeof(float4));
-                {
-                    Timed t(this, "kernel_render_ms");
-                    hip_check(launch_render_spline(ss, B0, sp->rendered.as<float4>(), stream), "render kernel");
-                    t.stop();
-                }
-                const FixedSet fs{sp->rendered.as<float4>(), (int)sp->n};
-                Timed t(this, "spline_projection_kernel_ms");
-                if (direct_mode) {
-                    ProjLaunch cfg = proj_launch(sp->n);
-                    cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
-                    hip_check(launch_project_fixed(c, fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
-                              "spline projection kernel");
-                } else {
-                    TileLaunch cfg = tile_launch(sp->nchunks);
-                    cfg.atomic |= force_atomic ? 1 : 0;
-                    hip_check(launch_project_tile(c, cull_params(c), &fs, nullptr, sp->boxes.as<ChunkBox>(),
-                                                  sp->nchunks, bins, arc, phase_delay, lut_profile, shared, cfg,
-                                                  stream),
-                              "spline projection kernel");
-                }
-                t.stop();
-            } else {
-                Timed t(this, "spline_projection_kernel_ms");
-                if (direct_mode) {
-                    ProjLaunch cfg = proj_launch(sp->n);
-                    cfg.atomic = (cfg.slices > 1 || force_atomic) ? 1 : 0;
-                    hip_check(launch_project_spline(c, ss, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
-                              "spline projection kernel");
-                } else {
-                    TileLaunch cfg = tile_launch(sp->nchunks);
-                    cfg.atomic |= force_atomic ? 1 : 0;
-                    hip_check(launch_project_tile(c, cull_params(c), nullptr, &ss, sp->boxes.as<ChunkBox>(),
-                                                  sp->nchunks, bins, arc, phase_delay, lut_profile, false, cfg,
-                                                  stream),
-                              "spline projection kernel");
-                }
-                t.stop();
-            }
-        }
-        const ConvParams p = conv_params(d_out);
+        ConvParams p = conv_params(d_out);
+        p.radix = frame_radix;
         {
             Timed t(this, "kernel_convolve_demodulate_ms");
             hip_check(launch_convolve_demod(p, stream), "convolve/demodulate kernel");
@@ -1044,6 +1179,7 @@ struct bcsim_sim {
         p.noise_amplitude = noise_amplitude > 0.0f ? noise_amplitude : 0.0f;
         p.noise_seed = noise_seed;
         p.noise_counter = noise_counter++;
+        p.radix = kDefaultRadix;
         return p;
     }
 

@@ -71,9 +71,15 @@ class RfSimulator:
         check(self._lib.bcsim_set_parameter(self._h, str(key).encode(), str(value).encode()))
 
     def get_parameter(self, key: str) -> str:
-        buf = C.create_string_buffer(512)
-        check(self._lib.bcsim_get_parameter(self._h, str(key).encode(), buf, len(buf)))
-        return buf.value.decode()
+        size = 512
+        while True:
+            buf = C.create_string_buffer(size)
+            status = self._lib.bcsim_get_parameter(self._h, str(key).encode(), buf, len(buf))
+            if status == 5 and size < (1 << 24):  # BCSIM_ERR_CAPACITY: long values (launch plans, tuning reports)
+                size *= 8
+                continue
+            check(status)
+            return buf.value.decode()
 
     def clear_fixed_scatterers(self):
         check(self._lib.bcsim_clear_fixed_scatterers(self._h))
