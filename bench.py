@@ -169,6 +169,96 @@ def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
             "first_frame_plan_and_tune_s": round(tune_s, 3)}
 
 
+def run_serving(seconds: float = 10.0, rate: float = 200.0, scale: float = 1.0, seed: int = 11):
+    """Serving loop: 16 resident phantoms (spline M-mode, fixed 2D, 1M-point LUT
+    volumes, uneven batched-mix jobs), geometries of 32-512 sector lines and the
+    4096-line pyramid; Poisson arrivals at ``rate`` requests/s for ``seconds``,
+    each naming a phantom, a geometry and a time t.  Every (phantom, geometry)
+    pair is requested once before the timed run (shape tuning and buffer set-up
+    happen there).  About 1% of the timed requests are re-simulated alone on a
+    single simulator and compared (rel L2)."""
+    import threading
+    from openbcsim_amd import workloads as W
+    from openbcsim_amd.pyrfsim import RfSimulator
+    from openbcsim_amd.serve import Server
+
+    phantoms, geometries, pairs = W.serving_mix(seed=5, scale=scale)
+    sims = []
+    for wl in phantoms:
+        s = RfSimulator("gpu")
+        wl.apply(s)
+        sims.append(s)
+    srv = Server()
+    for s in sims:
+        srv.add_phantom(s)
+    for (o, d, l, length) in geometries:
+        srv.add_geometry(o, d, l, length)
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    warm = [srv.submit(p, g, 0.5) for p, g in pairs]
+    for h in warm:
+        srv.wait(h)
+    warm_s = time.perf_counter() - t0
+    srv.set_parameter("reset_stats", 1)
+
+    n = max(1, int(rng.poisson(rate * seconds)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    choice = rng.integers(len(pairs), size=n)
+    times = rng.uniform(0.0, 1.0, n).astype(np.float32)
+    handles = [None] * n
+    ready = threading.Semaphore(0)
+    done = []
+
+    def waiter():
+        for i in range(n):
+            ready.acquire()
+            srv.wait(handles[i])
+            done.append(i)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    start = time.perf_counter()
+    late = 0.0
+    for i in range(n):
+        now = time.perf_counter() - start
+        if arrivals[i] > now:
+            time.sleep(arrivals[i] - now)
+        else:
+            late = max(late, now - arrivals[i])
+        p, g = pairs[int(choice[i])]
+        handles[i] = srv.submit(p, g, float(times[i]))
+        ready.release()
+    th.join()
+    wall = time.perf_counter() - start
+    lat = srv.stats("latency_ms")
+    jobs = srv.stats("frame_jobs")
+    lines = float(srv.stats("lines_completed")[0])
+    # ~1% of the requests again, alone on a single simulator
+    sample = rng.choice(n, max(1, n // 100), replace=False)
+    worst = 0.0
+    for i in sample:
+        p, g = pairs[int(choice[i])]
+        s = RfSimulator("gpu")
+        W.at_time(phantoms[p], geometries[g], float(times[i])).apply(s)
+        ref = s.simulate_frame()
+        s.close()
+        out = handles[i].out
+        worst = max(worst, float(np.linalg.norm(out - ref) / max(np.linalg.norm(ref), 1e-30)))
+    srv.close()
+    for s in sims:
+        s.close()
+    return {"requests": n, "seconds": round(wall, 3), "offered_rate_per_s": rate,
+            "achieved_rate_per_s": round(n / wall, 1), "max_submit_lag_ms": round(late * 1e3, 2),
+            "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_latency_ms": round(float(np.mean(lat)), 3),
+            "iq_lines_per_s": round(lines / wall, 1), "frames": int(len(jobs)),
+            "mean_requests_per_frame": round(float(np.mean(jobs)), 2),
+            "phantoms": len(phantoms), "geometries": len(geometries), "pairs": len(pairs),
+            "warmup_s": round(warm_s, 2),
+            "sampled_rel_err_vs_single": {"requests": int(len(sample)), "max": worst}}
+
+
 def cpu_baseline(wl, target_s: float = 12.0):
     """Reference CPU algorithm on the first lines of the scan (all scatterers)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -214,6 +304,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched-vs-per-call leg")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scatterer-count scale of the 64 jobs")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the timed serving run")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--tile", default=None, help="projection launch shape 'windows,lines,block' (default: automatic)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy kernel of "
@@ -270,6 +363,8 @@ def main():
     }
     if not args.no_batch:
         line["batched_64_jobs"] = run_batched(64, args.batch_scale)
+    if not args.no_serve:
+        line["serving"] = run_serving(args.serve_seconds, args.serve_rate)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(line), flush=True)

@@ -200,6 +200,59 @@ BCSIM_API bcsim_status bcsim_batch_synchronize(bcsim_batch* batch);
 BCSIM_API bcsim_status bcsim_batch_get_debug_data(bcsim_batch* batch, const char* key, double* out,
                                                   size_t capacity, size_t* count);
 
+/* ------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, whose frames are one
+ * simulate_lines call each, GpuAlgorithm.cpp:189-360; its GUI packet loop,
+ * MainWindow.cpp:666-695, is the usage pattern served here).
+ *
+ * A server keeps phantoms resident in device memory and takes frame requests
+ * (phantom, scan geometry, time t) from any thread.  A scheduler thread admits
+ * requests in arrival order while their frame buffers fit the memory budget and
+ * launches every admitted request as one batched frame; the next batch is formed
+ * and launched while the previous one runs (two frames in flight).  A request's
+ * IQ equals what a simulator of that phantom with that scan (all timestamps t)
+ * returns from simulate_lines, up to float summation order.
+ */
+typedef struct bcsim_server bcsim_server;
+
+BCSIM_API bcsim_status bcsim_server_create(bcsim_server** out);
+/* Fails requests that have not run; waits for frames in flight. */
+BCSIM_API bcsim_status bcsim_server_destroy(bcsim_server* server);
+/* Keys: memory_budget_bytes (device bytes for phantoms, frame buffers and
+ * outputs; default 90% of free device memory when the first phantom is added),
+ * max_jobs_per_frame (default 64), autotune (on/off, default on),
+ * reset_stats (any value: clears the statistics). */
+BCSIM_API bcsim_status bcsim_server_set_parameter(bcsim_server* server, const char* key, const char* value);
+/* Snapshot of a configured simulator (excitation, parameters, profile,
+ * scatterer datasets): its datasets stay resident, shared with the caller's
+ * simulator, which may be changed or destroyed afterwards. */
+BCSIM_API bcsim_status bcsim_server_add_phantom(bcsim_server* server, bcsim_sim* phantom, int* phantom_id);
+/* A scan geometry: num_lines origins, directions and lateral directions
+ * (float[3] each, as bcsim_set_scan_sequence) and the line length in metres. */
+BCSIM_API bcsim_status bcsim_server_add_geometry(bcsim_server* server, int num_lines, const float* origins,
+                                                 const float* directions, const float* lateral_dirs,
+                                                 float line_length, int* geometry_id);
+BCSIM_API bcsim_status bcsim_server_get_output_shape(bcsim_server* server, int phantom_id, int geometry_id,
+                                                     int* num_lines, int* num_samples);
+/* Queue one frame; out (complex64 [lines][samples], capacity complex elements)
+ * must stay valid until bcsim_server_wait reports the request. */
+BCSIM_API bcsim_status bcsim_server_submit(bcsim_server* server, int phantom_id, int geometry_id, float t,
+                                           float* out, size_t capacity, uint64_t* request_id);
+/* Wait (timeout_ms < 0: without limit) for a request; returns its status and
+ * releases its record.  The IQ, staged in pinned memory when the frame ends, is
+ * copied into the caller's buffer here, in the calling thread.  latency_ms is
+ * the time from submit to the IQ in the caller's buffer for a caller that
+ * waits from submit on (staging time plus this copy). */
+BCSIM_API bcsim_status bcsim_server_wait(bcsim_server* server, uint64_t request_id, int timeout_ms,
+                                         int* num_lines, int* num_samples, double* latency_ms);
+/* Keys: latency_ms, queue_ms (submit to launch), copy_ms (the copy in wait)
+ * per waited request;
+ * frame_jobs, frame_lines per launched frame; lines_completed, busy_span_ms
+ * (first submit to last completion), budget_bytes, used_bytes, resident_bytes,
+ * idle_slots, pending (one value each).  out NULL: count only. */
+BCSIM_API bcsim_status bcsim_server_get_stats(bcsim_server* server, const char* key, double* out,
+                                              size_t capacity, size_t* count);
+
 #ifdef __cplusplus
 }
 #endif

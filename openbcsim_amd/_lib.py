@@ -54,6 +54,15 @@ SIGNATURES = {
     "bcsim_batch_simulate": [_h, _fp, C.c_size_t],
     "bcsim_batch_synchronize": [_h],
     "bcsim_batch_get_debug_data": [_h, C.c_char_p, _dp, C.c_size_t, _sp],
+    "bcsim_server_create": [C.POINTER(C.c_void_p)],
+    "bcsim_server_destroy": [_h],
+    "bcsim_server_set_parameter": [_h, C.c_char_p, C.c_char_p],
+    "bcsim_server_add_phantom": [_h, _h, _ip],
+    "bcsim_server_add_geometry": [_h, C.c_int, _fp, _fp, _fp, C.c_float, _ip],
+    "bcsim_server_get_output_shape": [_h, C.c_int, C.c_int, _ip, _ip],
+    "bcsim_server_submit": [_h, C.c_int, C.c_int, C.c_float, _fp, C.c_size_t, C.POINTER(C.c_uint64)],
+    "bcsim_server_wait": [_h, C.c_uint64, C.c_int, _ip, _ip, _dp],
+    "bcsim_server_get_stats": [_h, C.c_char_p, _dp, C.c_size_t, _sp],
 }
 
 _lib = None

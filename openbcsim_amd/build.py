@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={A
           "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include"), "-fvisibility=hidden"]
 
-SOURCES = ["sim_kernels.hip", "project_tile.hip", "sim_host.cpp", "batch_host.cpp"]
+SOURCES = ["sim_kernels.hip", "project_tile.hip", "sim_host.cpp", "batch_host.cpp", "serve_host.cpp"]
 
 
 def _run(cmd):
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,308 +1,5 @@
-// Batched frames: many independent simulators (jobs) of uneven size simulated by
-// one persistent projection launch and one persistent convolution launch.
-//
-// The reference has no batched path: every simulate_lines call is its own
-// sequence of per-line launches (GpuAlgorithm.cpp:189-360).  Small jobs cannot
-// fill 256 CUs on their own, so running many of them as separate calls leaves
-// the GPU mostly idle; here each job keeps its own resident phantom, tables and
-// bins, and the batch
-//   1. zeroes the bins of jobs that accumulate and resets the work counters
-//      (one fill kernel, no memset nodes, so the sequence can be captured),
-//   2. renders all-timestamps-equal spline sets,
-//   3. runs one persistent projection kernel over (job, tile-block) items,
-//      heaviest items first,
-//   4. runs one persistent convolution kernel over (job, line) items, longest
-//      transforms first, writing each job's IQ at its offset of the output.
-// The sequence is captured once into a HIP graph and replayed while no job's
-// configuration and the output pointer stay the same.
-#include "sim_state.h"
-
-#include <numeric>
-
-namespace {
-
-struct JobPlan {
-    bcsim_sim* sim;
-    uint64_t version;
-    size_t out_offset;  // complex elements
-    int lines, samples;
-};
-
-}  // namespace
-
-struct bcsim_batch {
-    int device = -1;
-    hipStream_t stream = nullptr;
-    bool use_graph = true;
-    bool store_details = false;
-    std::vector<bcsim_sim*> sims;
-
-    // plan of the last build
-    bool planned = false;
-    std::vector<JobPlan> plan;
-    size_t total_out = 0;
-    DevBuf d_proj_jobs, d_proj_items, d_conv_jobs, d_conv_items, d_fill, d_counters;
-    int n_proj_items = 0, n_conv_items = 0, n_fill = 0;
-    size_t proj_lds = 0;
-    struct Render {
-        SplineSet set;
-        SplineLineBasis basis;
-        float4* out;
-    };
-    std::vector<Render> renders;
-    int workgroups = 256;
-    bool autotune = true;          // tune each job's shape class before its first batched frame
-    int tile_lines = 0;            // lines per tile for every job; 0 = each job's plan
-    int items_per_workgroup = 32;  // projection items per persistent workgroup (balance vs. overhead)
-
-    // graph of the last capture
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    void* graph_out = nullptr;
-
-    std::map<std::string, std::vector<double>> debug;
-
-    ~bcsim_batch() {
-        drop_graph();
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
-
-    void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-        graph_out = nullptr;
-    }
-
-    void invalidate() {
-        planned = false;
-        drop_graph();
-    }
-
-    bool stale() const {
-        if (!planned || plan.size() != sims.size()) return true;
-        for (size_t i = 0; i < sims.size(); i++)
-            if (plan[i].sim != sims[i] || plan[i].version != sims[i]->version) return true;
-        return false;
-    }
-
-    void add(bcsim_sim* s) {
-        need(s);
-        if (device < 0) device = s->device;
-        if (s->device != device) fail(BCSIM_ERR_INVALID_ARGUMENT, "all simulators of a batch must use one device");
-        sims.push_back(s);
-        invalidate();
-    }
-
-    // Chunk slices per projection job for the persistent grid: items of about
-    // equal estimated cost (chunks x lines, splines evaluated per line weighted
-    // x3), `items_per_workgroup` per workgroup, so the heaviest-first order
-    // balances without cutting light jobs into slices that only add atomics.
-    void replan_slices(std::vector<ProjJob>& pjobs, const std::vector<double>& cost) const {
-        double total = 0.0;
-        for (double c : cost) total += c;
-        const double per_item = total / ((double)workgroups * items_per_workgroup);
-        for (size_t k = 0; k < pjobs.size(); k++) {
-            ProjJob& J = pjobs[k];
-            const double want_items = std::max(1.0, cost[k] / std::max(per_item, 1.0));
-            int slices = (int)std::ceil(want_items / J.blocks_per_slice);
-            slices = std::max(1, std::min(slices, std::max(1, J.nchunks / 16)));
-            J.g.chunks_per_slice = std::max(1, (J.nchunks + slices - 1) / slices);
-            J.g.slices = std::max(1, (J.nchunks + J.g.chunks_per_slice - 1) / J.g.chunks_per_slice);
-        }
-    }
-
-    // Build the device-side job and item tables from the simulators' current state.
-    void build() {
-        if (sims.empty()) fail(BCSIM_ERR_INVALID_STATE, "empty batch");
-        hip_check(hipSetDevice(device), "hipSetDevice");
-        if (!stream) hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // frames in flight use the tables
-        drop_graph();
-        table_out = nullptr;
-        hipDeviceProp_t prop;
-        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-        workgroups = std::max(1, prop.multiProcessorCount);
-        plan.clear();
-        renders.clear();
-        std::vector<ProjJob> pjobs;
-        std::vector<double> pcost;  // per-item cost estimate of each projection job
-        std::vector<ConvParams> cjobs;
-        std::vector<FillSeg> fills;
-        std::vector<int> job_first_target, job_targets;
-        size_t offset = 0;
-        proj_lds = 0;
-        for (bcsim_sim* s : sims) {
-            s->prepare();
-            JobPlan jp{s, s->version, offset, (int)s->lines.size(), s->n_out};
-            plan.push_back(jp);
-            offset += (size_t)jp.lines * jp.samples;
-            const bool cplx = s->phase_delay;
-            const bool lut = s->profile == kProfileLut;
-            const bool shared = s->shared_origin();
-            const std::vector<bcsim_sim::ProjTarget> targets = s->proj_targets();
-            const bool tune_now = autotune || s->autotune;
-            if (tune_now) {
-                // tuning trials project this job's data: rendered sets must hold points
-                s->enqueue_renders();
-            }
-            for (const auto& t : targets) {
-                const bool is_spline = std::strcmp(t.kind, "spline") == 0;
-                ProjJob J{};
-                J.c = t.c;
-                J.cp = s->cull_params(t.c);
-                J.fs = t.fs;
-                J.ss = t.ss;
-                J.boxes = t.boxes;
-                J.bins = s->d_bins.as<float>();
-                J.g = s->plan_tile(t, tune_now);
-                if (tile_lines > 0) {
-                    TileTune tt = s->tune;
-                    tt.lines = tile_lines;
-                    J.g = s->tile_launch(t.nchunks, tt);
-                }
-                J.g.block = 1024;
-                J.nchunks = t.nchunks;
-                J.flags = (s->arc ? kJobArc : 0) | (cplx ? kJobPhase : 0) | (lut ? kJobLut : 0) |
-                          (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0);
-                const int tiles = (t.c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
-                J.blocks_per_slice = tiles * J.g.windows;
-                pjobs.push_back(J);
-                pcost.push_back((double)t.nchunks * t.c.num_lines * (is_spline ? 3.0 : 1.0));
-                if (std::strcmp(t.kind, "rendered") == 0) {
-                    Render r{t.ss, {}, t.sp->rendered.as<float4>()};
-                    s->host_basis_line0(*t.sp, r.basis);
-                    renders.push_back(r);
-                }
-            }
-            job_first_target.push_back((int)pjobs.size() - (int)targets.size());
-            job_targets.push_back((int)targets.size());
-            cjobs.push_back(s->conv_params(nullptr));  // iq pointer set per call (offset into the output)
-            cjobs.back().radix = s->plan_radix(tune_now);
-            s->quiesce();  // renders and trials on the job's own stream are done before the batch runs
-        }
-        replan_slices(pjobs, pcost);
-        // bins that are accumulated (or not written at all) start from zero
-        for (size_t k = 0; k < sims.size(); k++) {
-            bcsim_sim* s = sims[k];
-            const int first = job_first_target[k], n = job_targets[k];
-            bool single_store = n == 1 && pjobs[first].g.slices == 1;
-            for (int q = first; q < first + n; q++) pjobs[q].g.atomic = single_store ? 0 : 1;
-            if (!single_store)
-                fills.push_back(FillSeg{s->d_bins.as<float>(),
-                                        s->lines.size() * (size_t)s->num_rf * (s->phase_delay ? 2 : 1)});
-        }
-        for (const ProjJob& J : pjobs)
-            proj_lds = std::max(proj_lds, tile_lds_bytes(J.g.window_bins, (J.flags & kJobPhase) != 0, J.g.tile_lines,
-                                                         J.g.list_cap));
-        total_out = offset;
-
-        // items, heaviest first
-        std::vector<int> porder(pjobs.size());
-        std::iota(porder.begin(), porder.end(), 0);
-        std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return pcost[a] > pcost[b]; });
-        std::vector<int2> pitems;
-        for (int j : porder) {
-            const int n = pjobs[j].blocks_per_slice * pjobs[j].g.slices;
-            for (int i = 0; i < n; i++) pitems.push_back(make_int2(j, i));
-        }
-        std::vector<int> corder(cjobs.size());
-        std::iota(corder.begin(), corder.end(), 0);
-        std::stable_sort(corder.begin(), corder.end(),
-                         [&](int a, int b) { return cjobs[a].fft_len > cjobs[b].fft_len; });
-        std::vector<int2> citems;
-        for (int j : corder)
-            for (int l = 0; l < cjobs[j].num_lines; l++) citems.push_back(make_int2(j, l));
-
-        auto upload = [&](DevBuf& d, const void* src, size_t bytes) {
-            d.ensure(std::max<size_t>(bytes, 16));
-            if (bytes) hip_check(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
-        };
-        upload(d_proj_jobs, pjobs.data(), pjobs.size() * sizeof(ProjJob));
-        upload(d_proj_items, pitems.data(), pitems.size() * sizeof(int2));
-        upload(d_fill, fills.data(), fills.size() * sizeof(FillSeg));
-        upload(d_conv_items, citems.data(), citems.size() * sizeof(int2));
-        d_conv_jobs.ensure(std::max<size_t>(cjobs.size() * sizeof(ConvParams), 16));
-        d_counters.ensure(2 * sizeof(int));
-        conv_host = cjobs;
-        n_proj_items = (int)pitems.size();
-        n_conv_items = (int)citems.size();
-        n_fill = (int)fills.size();
-        planned = true;
-    }
-    std::vector<ConvParams> conv_host;
-
-    // Point each job's IQ at its slice of `out` (device memory).  Waits for frames
-    // in flight first: they read the job table being rewritten.
-    float2* table_out = nullptr;
-    void set_output(float2* out) {
-        if (table_out == out) return;
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        std::vector<ConvParams> cj = conv_host;
-        for (size_t i = 0; i < cj.size(); i++) cj[i].iq = out + plan[i].out_offset;
-        hip_check(hipMemcpy(d_conv_jobs.p, cj.data(), cj.size() * sizeof(ConvParams), hipMemcpyHostToDevice),
-                  "hipMemcpy");
-        table_out = out;
-    }
-
-    void enqueue() {
-        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), 2, stream), "fill kernel");
-        for (const Render& r : renders)
-            hip_check(launch_render_spline(r.set, r.basis, r.out, stream), "render kernel");
-        hip_check(launch_project_batch(d_proj_jobs.as<ProjJob>(), d_proj_items.as<int2>(), n_proj_items,
-                                       d_counters.as<int>(), proj_lds, workgroups, stream),
-                  "batched projection kernel");
-        hip_check(launch_convolve_batch(d_conv_jobs.as<ConvParams>(), d_conv_items.as<int2>(), n_conv_items,
-                                        d_counters.as<int>() + 1, workgroups, stream),
-                  "batched convolution kernel");
-    }
-
-    void simulate_device(void* d_out, size_t capacity) {
-        if (stale()) build();
-        if (capacity < total_out) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
-        hipEvent_t a = nullptr, b = nullptr;
-        if (store_details) {
-            hip_check(hipEventCreate(&a), "hipEventCreate");
-            hip_check(hipEventCreate(&b), "hipEventCreate");
-            hip_check(hipEventRecord(a, stream), "hipEventRecord");
-        }
-        if (use_graph) {
-            if (!exec || graph_out != d_out) {
-                drop_graph();
-                set_output(static_cast<float2*>(d_out));
-                hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                try {
-                    enqueue();
-                } catch (...) {
-                    hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(stream, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
-                hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                graph_out = d_out;
-            }
-            hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
-        } else {
-            set_output(static_cast<float2*>(d_out));
-            enqueue();
-        }
-        if (store_details) {
-            hip_check(hipEventRecord(b, stream), "hipEventRecord");
-            hip_check(hipEventSynchronize(b), "hipEventSynchronize");
-            float ms = 0.f;
-            hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-            debug["batch_frame_ms"].push_back(ms);
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-        }
-    }
-};
+// C-ABI of batched frames (bcsim_batch_*); the batch itself is in batch_state.h.
+#include "batch_state.h"
 
 extern "C" {
 

@@ -1,0 +1,696 @@
+// Serving loop: frame requests against resident phantoms, admitted by a device
+// memory budget and batched continuously.
+//
+// The reference serves one frame per simulate_lines call from the caller's
+// thread (GpuAlgorithm.cpp:189-360; the GUI's packet loop, MainWindow.cpp:666-695,
+// reconfigures the scan and calls again).  Here a server holds phantoms
+// (snapshots of configured simulators whose datasets and LUT stay in HBM) and
+// scan geometries, and takes frame requests (phantom, geometry, time t) from
+// any thread:
+//   - a scheduler thread admits pending requests in arrival order while their
+//     frame buffers fit the memory budget (buffers of finished requests are kept
+//     for the next request of the same phantom and geometry, or evicted least
+//     recently used first), and launches the admitted requests as one batched
+//     frame (one persistent projection launch, one persistent convolution
+//     launch) on one of two frame streams, so the next batch is formed and
+//     launched while the previous one runs;
+//   - each request's IQ is copied by DMA into its own pinned staging buffer at
+//     the end of its frame; a completion thread waits for frames in launch
+//     order and releases their frame buffers, which admits waiting requests;
+//   - the caller's wait copies the staged IQ into the caller's buffer, in the
+//     caller's thread (several threads for large frames), so one large result
+//     does not hold up the completion of the frames behind it.
+// Each request has its own bins and tables, so its result does not depend on
+// what it is batched with, up to float summation order.
+#include "batch_state.h"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <set>
+#include <thread>
+
+namespace bcsim_hip {
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Geometry {
+    std::vector<float> o, d, lat, t;
+    float length = 0.f;
+    int lines() const { return (int)t.size(); }
+};
+
+// Frame buffers of one in-flight or idle request: a simulator sharing its
+// phantom's datasets, with the geometry's scan.
+struct Slot {
+    std::unique_ptr<bcsim_sim> sim;
+    int phantom = -1, geometry = -1;
+    size_t bytes = 0;  // accounted device bytes
+    bool busy = false;
+    uint64_t last_used = 0;
+};
+
+// Pinned host buffers by power-of-two size class, reused across requests
+// (pinned allocation is slow and may stall the device).
+class PinnedPool {
+  public:
+    ~PinnedPool() {
+        for (auto& kv : free_)
+            for (void* p : kv.second) (void)hipHostFree(p);
+    }
+    void* take(size_t bytes, size_t& cls) {
+        cls = 4096;
+        while (cls < bytes) cls <<= 1;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto& v = free_[cls];
+            if (!v.empty()) {
+                void* p = v.back();
+                v.pop_back();
+                pooled_ -= cls;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        hip_check(hipHostMalloc(&p, cls, 0), "hipHostMalloc");
+        return p;
+    }
+    void give(void* p, size_t cls) {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu_);
+        if (pooled_ + cls > kMaxPooled) {
+            (void)hipHostFree(p);
+            return;
+        }
+        free_[cls].push_back(p);
+        pooled_ += cls;
+    }
+
+  private:
+    static constexpr size_t kMaxPooled = (size_t)8 << 30;
+    std::mutex mu_;
+    std::map<size_t, std::vector<void*>> free_;
+    size_t pooled_ = 0;
+};
+
+// memcpy split over a few threads for large copies (page faults of a fresh
+// destination are taken in parallel too).
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kChunk = (size_t)4 << 20;
+    const int parts = (int)std::min<size_t>(8, bytes / kChunk);
+    if (parts <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t step = (bytes + parts - 1) / parts;
+    std::vector<std::thread> th;
+    for (int i = 1; i < parts; i++) {
+        const size_t off = (size_t)i * step;
+        const size_t n = std::min(step, bytes - std::min(bytes, off));
+        th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, n); });
+    }
+    std::memcpy(dst, src, std::min(step, bytes));
+    for (auto& t : th) t.join();
+}
+
+struct Request {
+    uint64_t id = 0;
+    int phantom = 0, geometry = 0;
+    float t = 0.f;
+    float* out = nullptr;  // caller's complex64 [lines][samples]
+    size_t capacity = 0;
+    int lines = 0, samples = 0;
+    double t_submit = 0, t_launch = 0, t_done = 0;  // t_done: IQ staged in pinned memory
+    int status = BCSIM_OK;
+    std::string error;
+    bool done = false;
+    Slot* slot = nullptr;
+    void* staged = nullptr;  // pinned copy of the IQ
+    size_t staged_cls = 0;
+};
+
+struct Shape {
+    int lines = 0, samples = 0;
+    size_t slot_bytes = 0;
+};
+
+// One frame context: a batch, its device output, and the event that marks the
+// frame's IQ staged in pinned host memory.
+struct Frame {
+    bcsim_batch batch;
+    DevBuf d_out;
+    size_t out_accounted = 0;  // bytes of d_out counted in the server's `used`
+    hipEvent_t done = nullptr;
+    std::vector<Request*> reqs;
+    bool busy = false;
+    ~Frame() {
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
+}  // namespace
+}  // namespace bcsim_hip
+
+struct bcsim_server {
+    int device = -1;
+    size_t budget = 0;       // device bytes for phantoms, frame buffers and outputs
+    size_t used = 0;         // accounted device bytes
+    size_t resident = 0;     // phantom datasets and LUTs (part of `used`)
+    int max_jobs = 64;       // requests per batched frame
+    bool autotune = true;
+
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_comp, cv_done;
+    bool stop = false;
+    bool sched_exited = false;
+    uint64_t next_id = 1;
+    uint64_t tick = 0;
+    uint64_t completions = 0;
+
+    PinnedPool pinned;
+    std::vector<std::unique_ptr<bcsim_sim>> phantoms;  // snapshots (views) of the callers' simulators
+    std::set<const void*> resident_sets;               // datasets and LUTs already accounted
+    std::vector<Geometry> geometries;
+    std::map<std::pair<int, int>, Shape> shapes;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::deque<Request*> pending;
+    std::map<uint64_t, std::unique_ptr<Request>> requests;
+    Frame frames[2];
+    std::deque<Frame*> inflight;
+
+    // statistics (completion order)
+    std::vector<double> lat_ms, queue_ms, copy_ms, frame_jobs, frame_lines;
+    double lines_done = 0;
+    double t_first_submit = -1, t_last_done = -1;
+
+    std::thread scheduler, completer;
+
+    ~bcsim_server() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_sched.notify_all();
+        cv_comp.notify_all();
+        if (scheduler.joinable()) scheduler.join();
+        if (completer.joinable()) completer.join();
+        for (auto& kv : requests) pinned.give(kv.second->staged, kv.second->staged_cls);
+    }
+
+    void start() {
+        scheduler = std::thread([this] { scheduler_loop(); });
+        completer = std::thread([this] { completer_loop(); });
+    }
+
+    // ------------------------------------------------------------ registration
+    // Account the datasets and LUT of a phantom not yet counted (shared ones once).
+    void account_resident(const bcsim_sim& s) {
+        std::set<const void*> added;
+        size_t add = 0;
+        auto count = [&](const void* key, size_t bytes) {
+            if (!resident_sets.count(key) && added.insert(key).second) add += bytes;
+        };
+        if (s.lut) count(s.lut.get(), s.lut->bytes);
+        for (auto& f : s.fixed) count(f.get(), f->pts.bytes + f->boxes.bytes);
+        for (auto& sp : s.splines)
+            count(sp.get(), sp->cx.bytes + sp->cy.bytes + sp->cz.bytes + sp->amp.bytes + sp->boxes.bytes);
+        if (used + add > budget) fail(BCSIM_ERR_OUT_OF_MEMORY, "phantom datasets exceed the server's memory budget");
+        resident_sets.insert(added.begin(), added.end());
+        resident += add;
+        used += add;
+    }
+
+    int add_phantom(bcsim_sim* src) {
+        need(src);
+        if (!src->have_exc) fail(BCSIM_ERR_INVALID_STATE, "phantom has no excitation");
+        if (src->profile < 0) fail(BCSIM_ERR_INVALID_STATE, "phantom has no beam profile");
+        std::lock_guard<std::mutex> g(mu);
+        if (device < 0) device = src->device;
+        if (src->device != device) fail(BCSIM_ERR_INVALID_ARGUMENT, "all phantoms of a server must use one device");
+        if (budget == 0) {
+            hip_check(hipSetDevice(device), "hipSetDevice");
+            size_t free_b = 0, total_b = 0;
+            hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+            budget = (size_t)(0.9 * (double)free_b);
+        }
+        auto snap = std::make_unique<bcsim_sim>();
+        snap->configure_view_of(*src);
+        account_resident(*snap);
+        phantoms.push_back(std::move(snap));
+        return (int)phantoms.size() - 1;
+    }
+
+    int add_geometry(int L, const float* o, const float* d, const float* lat, float length) {
+        if (L <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "a geometry needs at least one line");
+        if (!o || !d || !lat) fail(BCSIM_ERR_INVALID_ARGUMENT, "null geometry array");
+        if (!(length > 0)) fail(BCSIM_ERR_INVALID_ARGUMENT, "line length must be positive");
+        Geometry gm;
+        gm.o.assign(o, o + 3 * (size_t)L);
+        gm.d.assign(d, d + 3 * (size_t)L);
+        gm.lat.assign(lat, lat + 3 * (size_t)L);
+        gm.t.assign(L, 0.0f);
+        gm.length = length;
+        std::lock_guard<std::mutex> g(mu);
+        geometries.push_back(std::move(gm));
+        return (int)geometries.size() - 1;
+    }
+
+    // Simulator for (phantom, geometry) with its frame shape known (nothing allocated).
+    std::unique_ptr<bcsim_sim> make_sim(int p, int gi) {
+        auto s = std::make_unique<bcsim_sim>();
+        s->configure_view_of(*phantoms[p]);
+        const Geometry& gm = geometries[gi];
+        s->set_scan_sequence(gm.lines(), gm.o.data(), gm.d.data(), gm.lat.data(), gm.t.data(), gm.length, true);
+        s->update_shape();
+        return s;
+    }
+
+    // Output shape and frame-buffer estimate of a pair (mu held).
+    const Shape& shape_of(int p, int gi) {
+        if (p < 0 || p >= (int)phantoms.size()) fail(BCSIM_ERR_INVALID_ARGUMENT, "unknown phantom");
+        if (gi < 0 || gi >= (int)geometries.size()) fail(BCSIM_ERR_INVALID_ARGUMENT, "unknown geometry");
+        auto key = std::make_pair(p, gi);
+        auto it = shapes.find(key);
+        if (it != shapes.end()) return it->second;
+        auto s = make_sim(p, gi);
+        Shape sh;
+        sh.lines = (int)s->lines.size();
+        sh.samples = s->n_out;
+        sh.slot_bytes = s->frame_bytes_estimate();
+        return shapes[key] = sh;
+    }
+
+    // ------------------------------------------------------------ requests
+    uint64_t submit(int p, int gi, float t, float* out, size_t capacity) {
+        if (!out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output buffer");
+        if (!std::isfinite(t)) fail(BCSIM_ERR_INVALID_ARGUMENT, "time must be finite");
+        std::lock_guard<std::mutex> g(mu);
+        if (stop) fail(BCSIM_ERR_INVALID_STATE, "server is shutting down");
+        const Shape& sh = shape_of(p, gi);
+        if (capacity < (size_t)sh.lines * sh.samples) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
+        auto r = std::make_unique<Request>();
+        r->id = next_id++;
+        r->phantom = p;
+        r->geometry = gi;
+        r->t = t;
+        r->out = out;
+        r->capacity = capacity;
+        r->lines = sh.lines;
+        r->samples = sh.samples;
+        r->t_submit = now_ms();
+        if (t_first_submit < 0) t_first_submit = r->t_submit;
+        pending.push_back(r.get());
+        const uint64_t id = r->id;
+        requests[id] = std::move(r);
+        cv_sched.notify_one();
+        return id;
+    }
+
+    // Wait for a request; its record is released once it has been reported.
+    void wait(uint64_t id, int timeout_ms, int* nl, int* ns, double* latency) {
+        std::unique_lock<std::mutex> lk(mu);
+        auto it = requests.find(id);
+        if (it == requests.end()) fail(BCSIM_ERR_INVALID_ARGUMENT, "unknown request id");
+        Request* r = it->second.get();
+        auto done = [&] { return r->done; };
+        if (timeout_ms < 0) cv_done.wait(lk, done);
+        else if (!cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+            fail(BCSIM_ERR_INVALID_STATE, "request not finished within the timeout");
+        if (nl) *nl = r->lines;
+        if (ns) *ns = r->samples;
+        const int status = r->status;
+        const std::string err = r->error;
+        std::unique_ptr<Request> own = std::move(it->second);
+        requests.erase(it);
+        if (status != BCSIM_OK) {
+            pinned.give(r->staged, r->staged_cls);
+            fail(status, err);
+        }
+        // copy out in this thread, without the lock
+        lk.unlock();
+        const double c0 = now_ms();
+        parallel_copy(r->out, r->staged, (size_t)r->lines * r->samples * sizeof(float2));
+        const double copy = now_ms() - c0;
+        pinned.give(r->staged, r->staged_cls);
+        const double lat = r->t_done - r->t_submit + copy;  // as seen by a caller waiting from submit
+        if (latency) *latency = lat;
+        lk.lock();
+        lat_ms.push_back(lat);
+        copy_ms.push_back(copy);
+    }
+
+    // ------------------------------------------------------------ scheduling (mu held)
+    Frame* free_frame() {
+        for (Frame& f : frames)
+            if (!f.busy) return &f;
+        return nullptr;
+    }
+
+    Slot* take_idle(int p, int gi) {
+        for (auto& s : slots)
+            if (!s->busy && s->phantom == p && s->geometry == gi) return s.get();
+        return nullptr;
+    }
+
+    // Free idle slots, least recently used first, until `need` more bytes fit.
+    bool make_room(size_t need, const Slot* keep) {
+        while (used + need > budget) {
+            Slot* lru = nullptr;
+            for (auto& s : slots)
+                if (!s->busy && s.get() != keep && (!lru || s->last_used < lru->last_used)) lru = s.get();
+            if (!lru) return false;
+            used -= lru->bytes;
+            for (size_t i = 0; i < slots.size(); i++)
+                if (slots[i].get() == lru) {
+                    slots.erase(slots.begin() + i);
+                    break;
+                }
+        }
+        return true;
+    }
+
+    void finish(Request* r, int status, const std::string& err, double t) {
+        r->status = status;
+        r->error = err;
+        r->done = true;
+        r->t_done = t;
+        if (r->slot) {
+            r->slot->busy = false;
+            r->slot->last_used = ++tick;
+            r->slot = nullptr;
+        }
+    }
+
+    void scheduler_loop() {
+        (void)hipSetDevice(device < 0 ? 0 : device);
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv_sched.wait(lk, [&] { return stop || (!pending.empty() && free_frame()); });
+            if (stop) break;
+            Frame* f = free_frame();
+            std::vector<Request*> admitted;
+            size_t out_elems = 0;
+            while (!pending.empty() && (int)admitted.size() < max_jobs) {
+                Request* r = pending.front();
+                const Shape& sh = shapes[std::make_pair(r->phantom, r->geometry)];
+                Slot* s = take_idle(r->phantom, r->geometry);
+                const size_t slot_need = s ? 0 : sh.slot_bytes;
+                const size_t out_bytes = (out_elems + (size_t)sh.lines * sh.samples) * sizeof(float2);
+                const size_t out_need = out_bytes > f->d_out.bytes ? out_bytes - f->d_out.bytes : 0;
+                if (!make_room(slot_need + out_need, s)) {
+                    if (admitted.empty() && inflight.empty()) {
+                        // nothing will free memory: this request can never run
+                        pending.pop_front();
+                        finish(r, BCSIM_ERR_OUT_OF_MEMORY, "request exceeds the server's memory budget", now_ms());
+                        cv_done.notify_all();
+                        continue;
+                    }
+                    break;
+                }
+                if (!s) {
+                    auto slot = std::make_unique<Slot>();
+                    try {
+                        slot->sim = make_sim(r->phantom, r->geometry);
+                    } catch (const BcsimError& e) {
+                        pending.pop_front();
+                        finish(r, e.code, e.what(), now_ms());
+                        cv_done.notify_all();
+                        continue;
+                    }
+                    slot->phantom = r->phantom;
+                    slot->geometry = r->geometry;
+                    slot->bytes = sh.slot_bytes;
+                    used += slot->bytes;
+                    s = slot.get();
+                    slots.push_back(std::move(slot));
+                }
+                s->busy = true;
+                r->slot = s;
+                pending.pop_front();
+                admitted.push_back(r);
+                out_elems += (size_t)sh.lines * sh.samples;
+            }
+            if (admitted.empty()) {
+                if (pending.empty()) continue;
+                const uint64_t seen = completions;  // wait until a frame finishes and frees memory
+                cv_sched.wait(lk, [&] { return stop || completions != seen; });
+                continue;
+            }
+            f->busy = true;
+            lk.unlock();
+            std::string err;
+            int code = BCSIM_OK;
+            try {
+                launch(f, admitted);
+            } catch (const BcsimError& e) {
+                code = e.code;
+                err = e.what();
+            } catch (const std::exception& e) {
+                code = BCSIM_ERR_INTERNAL;
+                err = e.what();
+            }
+            lk.lock();
+            if (code == BCSIM_OK && f->reqs.empty()) {  // every admitted request failed in preparation
+                f->busy = false;
+                continue;
+            }
+            if (code != BCSIM_OK) {
+                const double t = now_ms();
+                for (Request* r : f->reqs) finish(r, code, err, t);
+                f->reqs.clear();
+                f->busy = false;
+                cv_done.notify_all();
+                continue;
+            }
+            // account the frame buffers as allocated
+            for (Request* r : f->reqs) {
+                const size_t actual = r->slot->sim->frame_bytes();
+                used = used - r->slot->bytes + actual;
+                r->slot->bytes = actual;
+            }
+            inflight.push_back(f);
+            cv_comp.notify_one();
+        }
+        // fail what never launched
+        const double t = now_ms();
+        for (Request* r : pending) finish(r, BCSIM_ERR_INVALID_STATE, "server shut down before the request ran", t);
+        pending.clear();
+        sched_exited = true;
+        cv_comp.notify_all();
+        cv_done.notify_all();
+    }
+
+    // Prepare the admitted requests (outside mu; their slots are ours) and launch
+    // them as one batched frame followed by the copy to pinned memory.
+    void launch(Frame* f, std::vector<Request*>& admitted) {
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        f->reqs.clear();
+        for (Request* r : admitted) {
+            bcsim_sim* s = r->slot->sim.get();
+            try {
+                s->set_frame_time(r->t);
+                s->prepare();
+                f->reqs.push_back(r);
+            } catch (const BcsimError& e) {
+                std::lock_guard<std::mutex> g(mu);
+                finish(r, e.code, e.what(), now_ms());
+                cv_done.notify_all();
+            }
+        }
+        if (f->reqs.empty()) return;
+        for (Request* r : f->reqs) r->t_launch = now_ms();
+        bcsim_batch& b = f->batch;
+        b.use_graph = false;  // the mix changes from frame to frame
+        b.autotune = autotune;
+        b.sims.clear();
+        b.device = -1;
+        b.invalidate();
+        for (Request* r : f->reqs) b.add(r->slot->sim.get());
+        b.build();
+        const size_t total = b.total_out;
+        if (f->d_out.bytes < total * sizeof(float2)) {
+            hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+            f->d_out.ensure(total * sizeof(float2));
+        }
+        for (Request* r : f->reqs)
+            if (!r->staged) r->staged = pinned.take((size_t)r->lines * r->samples * sizeof(float2), r->staged_cls);
+        if (!f->done) hip_check(hipEventCreateWithFlags(&f->done, hipEventDisableTiming), "hipEventCreate");
+        b.simulate_device(f->d_out.p, total);
+        for (size_t i = 0; i < f->reqs.size(); i++) {
+            Request* r = f->reqs[i];
+            const float2* src = f->d_out.as<float2>() + b.plan[i].out_offset;
+            hip_check(hipMemcpyAsync(r->staged, src, (size_t)r->lines * r->samples * sizeof(float2),
+                                     hipMemcpyDeviceToHost, b.stream),
+                      "hipMemcpyAsync");
+        }
+        hip_check(hipEventRecord(f->done, b.stream), "hipEventRecord");
+        std::lock_guard<std::mutex> g(mu);
+        used = used - f->out_accounted + f->d_out.bytes;
+        f->out_accounted = f->d_out.bytes;
+        frame_jobs.push_back((double)f->reqs.size());
+        double nl = 0;
+        for (Request* r : f->reqs) nl += r->lines;
+        frame_lines.push_back(nl);
+    }
+
+    void completer_loop() {
+        (void)hipSetDevice(device < 0 ? 0 : device);
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv_comp.wait(lk, [&] { return !inflight.empty() || sched_exited; });
+            if (inflight.empty()) break;
+            Frame* f = inflight.front();
+            lk.unlock();
+            const hipError_t e = hipEventSynchronize(f->done);
+            const double t = now_ms();
+            lk.lock();
+            for (Request* r : f->reqs) {
+                if (e == hipSuccess) {
+                    finish(r, BCSIM_OK, "", t);
+                    queue_ms.push_back(r->t_launch - r->t_submit);
+                    lines_done += r->lines;
+                    t_last_done = t;
+                } else {
+                    finish(r, BCSIM_ERR_DEVICE, std::string("frame failed: ") + hipGetErrorString(e), t);
+                }
+            }
+            f->reqs.clear();
+            f->busy = false;
+            inflight.pop_front();
+            completions++;
+            cv_sched.notify_all();
+            cv_done.notify_all();
+        }
+    }
+
+    void get_stats(const std::string& key, std::vector<double>& out) {
+        std::lock_guard<std::mutex> g(mu);
+        if (key == "latency_ms") out = lat_ms;
+        else if (key == "queue_ms") out = queue_ms;
+        else if (key == "copy_ms") out = copy_ms;
+        else if (key == "frame_jobs") out = frame_jobs;
+        else if (key == "frame_lines") out = frame_lines;
+        else if (key == "lines_completed") out = {lines_done};
+        else if (key == "busy_span_ms") out = {t_first_submit >= 0 && t_last_done >= 0 ? t_last_done - t_first_submit : 0.0};
+        else if (key == "budget_bytes") out = {(double)budget};
+        else if (key == "used_bytes") out = {(double)used};
+        else if (key == "resident_bytes") out = {(double)resident};
+        else if (key == "idle_slots") {
+            double n = 0;
+            for (auto& s : slots) n += !s->busy;
+            out = {n};
+        } else if (key == "pending") out = {(double)pending.size()};
+        else fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid statistics key: " + key);
+    }
+
+};
+
+extern "C" {
+
+BCSIM_API bcsim_status bcsim_server_create(bcsim_server** out) {
+    return guarded([&] {
+        if (!out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output handle");
+        auto s = std::make_unique<bcsim_server>();
+        s->start();
+        *out = s.release();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_destroy(bcsim_server* s) {
+    return guarded([&] { delete s; });
+}
+
+BCSIM_API bcsim_status bcsim_server_set_parameter(bcsim_server* s, const char* key, const char* value) {
+    return guarded([&] {
+        if (!s || !key || !value) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        const std::string k = key;
+        std::lock_guard<std::mutex> g(s->mu);
+        if (k == "memory_budget_bytes") {
+            const long long v = std::stoll(value);
+            if (v <= 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "memory budget must be positive");
+            if ((size_t)v < s->used) fail(BCSIM_ERR_INVALID_ARGUMENT, "memory budget below the bytes already in use");
+            s->budget = (size_t)v;
+        } else if (k == "max_jobs_per_frame") {
+            const int v = parse_int(value);
+            if (v < 1) fail(BCSIM_ERR_INVALID_ARGUMENT, "max_jobs_per_frame must be >= 1");
+            s->max_jobs = v;
+        } else if (k == "autotune") {
+            s->autotune = parse_bool(value);
+        } else if (k == "reset_stats") {
+            s->lat_ms.clear();
+            s->queue_ms.clear();
+            s->copy_ms.clear();
+            s->frame_jobs.clear();
+            s->frame_lines.clear();
+            s->lines_done = 0;
+            s->t_first_submit = s->t_last_done = -1;
+        } else {
+            fail(BCSIM_ERR_INVALID_ARGUMENT, "illegal parameter name: '" + k + "'");
+        }
+        s->cv_sched.notify_all();
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_add_phantom(bcsim_server* s, bcsim_sim* phantom, int* phantom_id) {
+    return guarded([&] {
+        if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null server handle");
+        const int id = s->add_phantom(phantom);
+        if (phantom_id) *phantom_id = id;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_add_geometry(bcsim_server* s, int num_lines, const float* origins,
+                                                 const float* directions, const float* lateral_dirs,
+                                                 float line_length, int* geometry_id) {
+    return guarded([&] {
+        if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null server handle");
+        const int id = s->add_geometry(num_lines, origins, directions, lateral_dirs, line_length);
+        if (geometry_id) *geometry_id = id;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_get_output_shape(bcsim_server* s, int phantom_id, int geometry_id,
+                                                     int* num_lines, int* num_samples) {
+    return guarded([&] {
+        if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null server handle");
+        std::lock_guard<std::mutex> g(s->mu);
+        const Shape& sh = s->shape_of(phantom_id, geometry_id);
+        if (num_lines) *num_lines = sh.lines;
+        if (num_samples) *num_samples = sh.samples;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_submit(bcsim_server* s, int phantom_id, int geometry_id, float t, float* out,
+                                           size_t capacity, uint64_t* request_id) {
+    return guarded([&] {
+        if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null server handle");
+        const uint64_t id = s->submit(phantom_id, geometry_id, t, out, capacity);
+        if (request_id) *request_id = id;
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_wait(bcsim_server* s, uint64_t request_id, int timeout_ms, int* num_lines,
+                                         int* num_samples, double* latency_ms) {
+    return guarded([&] {
+        if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null server handle");
+        s->wait(request_id, timeout_ms, num_lines, num_samples, latency_ms);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_get_stats(bcsim_server* s, const char* key, double* out, size_t capacity,
+                                              size_t* count) {
+    return guarded([&] {
+        if (!s || !key) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        std::vector<double> v;
+        s->get_stats(key, v);
+        if (count) *count = v.size();
+        const size_t n = std::min(capacity, v.size());
+        if (out && n) std::memcpy(out, v.data(), n * sizeof(double));
+    });
+}
+
+}  // extern "C"

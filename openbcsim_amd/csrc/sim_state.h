@@ -67,7 +67,9 @@ struct DevBuf {
 // Datasets are stored in Morton (z-curve) order of position, in chunks of
 // kChunk scatterers with one bounding box each, so the tile kernel can skip
 // whole chunks that lie outside a beam.  The order of the scatterers does not
-// change the frame beyond float summation order.
+// change the frame beyond float summation order.  A dataset is immutable once
+// built and shared (shared_ptr) by every simulator that projects it: the
+// serving path keeps one resident copy of a phantom for many frame requests.
 struct FixedDataset {
     DevBuf pts;    // float4[n] (x, y, z, amplitude), Morton order
     DevBuf boxes;  // ChunkBox[nchunks]
@@ -83,9 +85,13 @@ struct SplineDataset {
     std::vector<float> knots;
     DevBuf cx, cy, cz, amp;  // [num_cp][n] x3, [n], Morton order of the control-point centroid
     DevBuf boxes;            // ChunkBox[nchunks] over all control points of the chunk
-    DevBuf basis;            // SplineLineBasis[num_lines]
-    DevBuf rendered;         // float4[n], all-timestamps-equal path (boxes: `boxes`, since a
-                             // rendered point lies in the box of its control points)
+};
+
+// Per-simulator buffers of one spline dataset that depend on the scan.
+struct SplineScan {
+    DevBuf basis;     // SplineLineBasis[num_lines]
+    DevBuf rendered;  // float4[n], all-timestamps-equal path (culled with the dataset's boxes,
+                      // since a rendered point lies in the box of its control points)
 };
 
 // 10 bits of each coordinate interleaved into a 30-bit z-curve key.
@@ -262,13 +268,14 @@ struct bcsim_sim {
     bool all_equal = false;
 
     // phantom
-    std::vector<std::unique_ptr<FixedDataset>> fixed;
-    std::vector<std::unique_ptr<SplineDataset>> splines;
+    std::vector<std::shared_ptr<const FixedDataset>> fixed;
+    std::vector<std::shared_ptr<const SplineDataset>> splines;
+    std::vector<std::unique_ptr<SplineScan>> spline_scan;  // parallel to `splines`
 
     // profile
     int profile = -1;
     float sigma_l = 0.f, sigma_e = 0.f;
-    DevBuf lut;  // float4 quads [n_rad][n_lat][n_ele]
+    std::shared_ptr<DevBuf> lut;  // float4 quads [n_rad][n_lat][n_ele]; replaced, never rewritten
     int lut_nr = 0, lut_nl = 0, lut_ne = 0;
     float lut_r0 = 0, lut_r1 = 0, lut_l0 = 0, lut_l1 = 0, lut_e0 = 0, lut_e1 = 0;
 
@@ -455,6 +462,80 @@ struct bcsim_sim {
         dirty_scan = dirty_conv = dirty_basis = true;
     }
 
+    // Become a simulator of the same phantom as `src`: copy its configuration
+    // (parameters, excitation, profile, scan) and share its resident datasets and
+    // LUT; scan-dependent buffers, bins and tables are this simulator's own.
+    void configure_view_of(const bcsim_sim& src) {
+        ++version;
+        verbose = src.verbose;
+        sound_speed = src.sound_speed;
+        noise_amplitude = src.noise_amplitude;
+        noise_seed = src.noise_seed;
+        arc = src.arc;
+        decimation = src.decimation;
+        phase_delay = src.phase_delay;
+        device = src.device;
+        direct_mode = src.direct_mode;
+        tune = src.tune;
+        autotune = src.autotune;
+        radix = src.radix;
+        have_exc = src.have_exc;
+        exc = src.exc;
+        center = src.center;
+        fs = src.fs;
+        f_demod = src.f_demod;
+        have_scan = src.have_scan;
+        lines = src.lines;
+        line_length = src.line_length;
+        all_equal = src.all_equal;
+        fixed = src.fixed;
+        splines = src.splines;
+        spline_scan.clear();
+        for (size_t i = 0; i < splines.size(); i++) spline_scan.push_back(std::make_unique<SplineScan>());
+        profile = src.profile;
+        sigma_l = src.sigma_l;
+        sigma_e = src.sigma_e;
+        lut = src.lut;
+        lut_nr = src.lut_nr, lut_nl = src.lut_nl, lut_ne = src.lut_ne;
+        lut_r0 = src.lut_r0, lut_r1 = src.lut_r1, lut_l0 = src.lut_l0;
+        lut_l1 = src.lut_l1, lut_e0 = src.lut_e0, lut_e1 = src.lut_e1;
+        can_change_device = false;
+        dirty_scan = dirty_conv = dirty_basis = true;
+    }
+
+    // Give every line the time t (a frame of a dynamic phantom at one instant):
+    // only the spline evaluation time changes, no table is rebuilt.
+    void set_frame_time(float t) {
+        ++version;
+        for (auto& q : lines) q.t = t;
+        all_equal = true;
+        dirty_basis = true;
+    }
+
+    // Device bytes this simulator holds for its frames (not the shared datasets).
+    size_t frame_bytes() const {
+        size_t b = d_lines.bytes + d_H.bytes + d_demod.bytes + d_tw.bytes + d_bins.bytes + d_iq.bytes;
+        for (auto& sc : spline_scan) b += sc->basis.bytes + sc->rendered.bytes;
+        return b;
+    }
+
+    // The same before any allocation, from the frame shape (update_shape() done).
+    size_t frame_bytes_estimate() const {
+        const size_t L = lines.size();
+        size_t b = L * sizeof(LineGeom) + (size_t)fft_len * sizeof(float2) * 2 + (size_t)n_out * sizeof(float2) +
+                   L * (size_t)num_rf * (phase_delay ? 8 : 4);
+        for (auto& sp : splines) b += sp->n * sizeof(float4) + (all_equal ? 0 : L * sizeof(SplineLineBasis));
+        return b;
+    }
+
+    // Device bytes of the datasets and LUT (shared by views).
+    size_t resident_bytes() const {
+        size_t b = lut ? lut->bytes : 0;
+        for (auto& f : fixed) b += f->pts.bytes + f->boxes.bytes;
+        for (auto& sp : splines) b += sp->cx.bytes + sp->cy.bytes + sp->cz.bytes + sp->amp.bytes + sp->boxes.bytes;
+        return b;
+    }
+
     void clear_fixed() {
         quiesce();
         ++version;
@@ -467,7 +548,7 @@ struct bcsim_sim {
         if (n > (size_t)0x7fffffff) fail(BCSIM_ERR_INVALID_ARGUMENT, "too many scatterers in one dataset");
         use_device();
         quiesce();
-        auto ds = std::make_unique<FixedDataset>();
+        auto ds = std::make_shared<FixedDataset>();
         ds->n = n;
         ds->nchunks = (int)((n + kChunk - 1) / kChunk);
         const std::vector<uint32_t> perm = morton_order(xyza, n, 4);
@@ -494,6 +575,7 @@ struct bcsim_sim {
         quiesce();
         ++version;
         splines.clear();
+        spline_scan.clear();
     }
 
     void add_spline(int degree, const float* knots, int n_knots, const float* cps, const float* amps, size_t n,
@@ -508,7 +590,7 @@ struct bcsim_sim {
         if (n > (size_t)0x7fffffff) fail(BCSIM_ERR_INVALID_ARGUMENT, "too many scatterers in one dataset");
         use_device();
         quiesce();
-        auto ds = std::make_unique<SplineDataset>();
+        auto ds = std::make_shared<SplineDataset>();
         ds->degree = degree;
         ds->num_cp = num_cp;
         ds->n = n;
@@ -556,6 +638,7 @@ struct bcsim_sim {
                       "hipMemcpy");
         }
         splines.push_back(std::move(ds));
+        spline_scan.push_back(std::make_unique<SplineScan>());
         dirty_basis = true;
         if (verbose) log(1, "Number of spline scatterers: " + std::to_string(total_spline()));
     }
@@ -588,8 +671,10 @@ struct bcsim_sim {
                 for (int e = 0; e < ne; e++)
                     quad[((size_t)r * nl + l) * ne + e] =
                         make_float4(at(r, l, e), at(r, l, e + 1), at(r, l + 1, e), at(r, l + 1, e + 1));
-        lut.ensure(cells * sizeof(float4));
-        hip_check(hipMemcpy(lut.p, quad.data(), cells * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
+        auto table = std::make_shared<DevBuf>();
+        table->ensure(cells * sizeof(float4));
+        hip_check(hipMemcpy(table->p, quad.data(), cells * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
+        lut = std::move(table);
         lut_nr = nr;
         lut_nl = nl;
         lut_ne = ne;
@@ -702,9 +787,13 @@ struct bcsim_sim {
     // (GpuAlgorithm.cpp:657-692 does this per line with a constant-memory upload).
     void upload_spline_basis() {
         const int L = (int)lines.size();
-        for (auto& sp : splines) {
+        for (size_t q = 0; q < splines.size(); q++) {
+            const SplineDataset* sp = splines[q].get();
             std::vector<SplineLineBasis> tab(std::max(L, 1));
-            for (int k = 0; k < L; k++) {
+            // all timestamps equal: the frame is rendered at line 0's time and the
+            // table is not read, so only that time is checked
+            const int checked = all_equal ? std::min(L, 1) : L;
+            for (int k = 0; k < checked; k++) {
                 const float t = lines[k].t;
                 const int mu = knot_interval(sp->knots, t);
                 const int lo = mu - sp->degree, hi = mu;
@@ -716,8 +805,10 @@ struct bcsim_sim {
                 B.count = hi - lo + 1;
                 for (int i = 0; i < B.count; i++) B.b[i] = bspline_basis(lo + i, sp->degree, t, sp->knots);
             }
-            sp->basis.ensure(sizeof(SplineLineBasis) * tab.size());
-            hip_check(hipMemcpy(sp->basis.p, tab.data(), sizeof(SplineLineBasis) * tab.size(),
+            if (all_equal) continue;
+            DevBuf& basis = spline_scan[q]->basis;
+            basis.ensure(sizeof(SplineLineBasis) * tab.size());
+            hip_check(hipMemcpy(basis.p, tab.data(), sizeof(SplineLineBasis) * tab.size(),
                                 hipMemcpyHostToDevice), "hipMemcpy");
         }
     }
@@ -768,7 +859,7 @@ struct bcsim_sim {
             P.inv_tsl = (float)(1.0 / (2.0 * (double)sigma_l * (double)sigma_l));
             P.inv_tse = (float)(1.0 / (2.0 * (double)sigma_e * (double)sigma_e));
         } else {
-            P.quad = lut.as<float4>();
+            P.quad = lut->as<float4>();
             P.nr = lut_nr;
             P.nl = lut_nl;
             P.ne = lut_ne;
@@ -937,7 +1028,8 @@ struct bcsim_sim {
         const ChunkBox* boxes = nullptr;
         int nchunks = 0;
         size_t n = 0;
-        SplineDataset* sp = nullptr;  // spline kinds
+        const SplineDataset* sp = nullptr;  // spline kinds
+        SplineScan* scan = nullptr;
     };
 
     std::vector<ProjTarget> proj_targets() {
@@ -951,19 +1043,22 @@ struct bcsim_sim {
             t.n = f->n;
             out.push_back(t);
         }
-        for (auto& sp : splines) {
+        for (size_t q = 0; q < splines.size(); q++) {
+            const SplineDataset* sp = splines[q].get();
+            SplineScan* scan = spline_scan[q].get();
             if (sp->n == 0) continue;
             ProjTarget t{all_equal ? "rendered" : "spline", proj_common(true)};
             t.ss = SplineSet{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
-                             sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
+                             scan->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
             if (all_equal) {
-                sp->rendered.ensure(sp->n * sizeof(float4));
-                t.fs = FixedSet{sp->rendered.as<float4>(), (int)sp->n};
+                scan->rendered.ensure(sp->n * sizeof(float4));
+                t.fs = FixedSet{scan->rendered.as<float4>(), (int)sp->n};
             }
             t.boxes = sp->boxes.as<ChunkBox>();
             t.nchunks = sp->nchunks;
             t.n = sp->n;
-            t.sp = sp.get();
+            t.sp = sp;
+            t.scan = scan;
             out.push_back(t);
         }
         return out;
@@ -973,17 +1068,18 @@ struct bcsim_sim {
     // (GpuAlgorithm.cpp:181-187, GpuScatterers.cpp:48-123).
     void enqueue_renders() {
         if (!all_equal) return;
-        for (auto& sp : splines) {
-            if (sp->n == 0) continue;
-            const SplineSet ss{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
-                               sp->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
-            SplineLineBasis B0;
-            host_basis_line0(*sp, B0);
-            sp->rendered.ensure(sp->n * sizeof(float4));
-            Timed t(this, "kernel_render_ms");
-            hip_check(launch_render_spline(ss, B0, sp->rendered.as<float4>(), stream), "render kernel");
-            t.stop();
+        for (const ProjTarget& t : proj_targets()) {
+            Timed tm(this, "kernel_render_ms");
+            render_target(t);
+            tm.stop();
         }
+    }
+
+    void render_target(const ProjTarget& t) {
+        if (std::strcmp(t.kind, "rendered") != 0) return;
+        SplineLineBasis B0;
+        host_basis_line0(*t.sp, B0);
+        hip_check(launch_render_spline(t.ss, B0, t.scan->rendered.as<float4>(), stream), "render kernel");
     }
 
     void launch_target(const ProjTarget& t, const TileLaunch& cfg, float* bins) {
@@ -1011,6 +1107,7 @@ struct bcsim_sim {
         TileTune best;
         if (TuneCache::get().find_tile(key, best)) return best;
         float* bins = d_bins.as<float>();
+        render_target(t);  // trials project the points of this frame's time
         std::string trials;
         auto cost = [&](const TileTune& tt) {
             const TileLaunch cfg = tile_launch(t.nchunks, tt);
@@ -1074,9 +1171,10 @@ struct bcsim_sim {
                       ceil_log2((long long)lines.size()));
         int r = 0;
         if (TuneCache::get().find_radix(key, r)) return r;
-        d_iq.ensure(lines.size() * (size_t)n_out * sizeof(float2));
+        DevBuf scratch;  // trial output, freed after the trials
+        scratch.ensure(lines.size() * (size_t)n_out * sizeof(float2));
         const uint32_t counter = noise_counter;  // trials do not advance the noise stream
-        ConvParams p = conv_params(d_iq.as<float2>());
+        ConvParams p = conv_params(scratch.as<float2>());
         noise_counter = counter;
         std::string trials;
         double best_ms = 0.0;

@@ -307,3 +307,53 @@ def batched_jobs(num_jobs: int = 64, seed: int = 3, scale: float = 1.0) -> List[
         wl.name = f"job{j}"
         jobs.append(wl)
     return jobs
+
+
+def at_time(wl: Workload, geometry: tuple, t: float) -> Workload:
+    """The frame a serving request (phantom ``wl``, scan ``geometry`` =
+    (origins, dirs, lats, line_length), time ``t``) stands for: the phantom
+    with that scan, every timestamp t."""
+    o, d, l, length = geometry
+    out = dataclasses.replace(wl)
+    out.scan = (o, d, l, np.full(o.shape[0], np.float32(t), np.float32), float(length))
+    out.all_timestamps_equal = True
+    return out
+
+
+def serving_mix(seed: int = 5, scale: float = 1.0):
+    """16 phantoms and the scan geometries of the serving workload, drawn from the
+    2D spline (M-mode-like), 3D LUT volume and uneven batched mixes:
+      phantoms: 6 cubic-spline phantoms (200k, 10 control points), 4 fixed 2D
+      sector phantoms (100k), 2 LUT volumes (1M), 4 uneven jobs of the batched
+      mix -- scatterer counts times ``scale``;
+      geometries: sector scans of 32-512 lines at the phantoms' depths and the
+      64x64 pyramid (4096 lines).
+    Returns (phantoms, geometries, pairs): ``pairs`` lists the (phantom,
+    geometry) combinations requests draw from (a geometry whose depth fits the
+    phantom's LUT extent)."""
+    rng = np.random.default_rng(seed)
+    phantoms: List[Workload] = []
+    for k in range(6):
+        phantoms.append(spline_mmode(max(16, int(200_000 * scale)), 64, seed=100 + k, decimation=4))
+    for k in range(4):
+        phantoms.append(fixed_sector_2d(max(16, int(100_000 * scale)), 64, seed=200 + k))
+    for k in range(2):
+        phantoms.append(volume_lut(max(16, int(1_000_000 * scale)), n_az=8, n_el=8, seed=300 + k))
+    for wl in batched_jobs(4, seed=400, scale=scale):
+        phantoms.append(wl)
+    geometries = []
+    for n in (32, 64, 128, 256, 512):
+        o, d, l, _, _ = sector_scan(n, math.radians(70.0), 0.12)
+        geometries.append((o, d, l, 0.12))
+    o, d, l, _, _ = pyramid_scan(64, 64, math.radians(30.0), math.radians(30.0), 0.15)
+    geometries.append((o, d, l, 0.15))
+    pairs = []
+    for p, wl in enumerate(phantoms):
+        for g, (o, d, l, length) in enumerate(geometries):
+            if wl.profile[0] == "lut" and length > wl.profile[1][1]:
+                continue  # beyond the LUT's radial extent the profile is zero
+            if o.shape[0] == 4096 and wl.name != "volume_lut":
+                continue  # the pyramid scans the 3D phantoms
+            pairs.append((p, g))
+    rng.shuffle(pairs)
+    return phantoms, geometries, pairs
