@@ -282,6 +282,7 @@ struct bcsim_sim {
     // derived device tables
     bool dirty_scan = true, dirty_conv = true, dirty_basis = true;
     int num_rf = 0, fft_len = 0, log2_len = 0, n_out = 0;
+    int tables_fft = 0, tables_out = 0;  // frame shape the convolution tables were built for
     DevBuf d_lines, d_H, d_demod, d_tw, d_bins, d_iq;
 
     std::map<std::string, std::vector<double>> debug;
@@ -459,7 +460,9 @@ struct bcsim_sim {
         all_equal = equal;
         have_scan = true;
         can_change_device = false;
-        dirty_scan = dirty_conv = dirty_basis = true;
+        // the convolution tables follow the frame shape; prepare() rebuilds them
+        // only if the new scan changes it
+        dirty_scan = dirty_basis = true;
     }
 
     // Become a simulator of the same phantom as `src`: copy its configuration
@@ -781,6 +784,8 @@ struct bcsim_sim {
         hip_check(hipMemcpy(d_H.p, H.data(), sizeof(float2) * N, hipMemcpyHostToDevice), "hipMemcpy");
         if (n_out) hip_check(hipMemcpy(d_demod.p, dem.data(), sizeof(float2) * n_out, hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMemcpy(d_tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        tables_fft = fft_len;
+        tables_out = n_out;
     }
 
     // Per spline dataset and line: knot interval and the p+1 float basis values
@@ -820,8 +825,10 @@ struct bcsim_sim {
         use_device();
         if (dirty_scan || dirty_conv || dirty_basis) quiesce();
         if (dirty_scan || dirty_conv) {
+            // update_shape() may already have run (bcsim_get_output_shape): compare the
+            // shape with the one the tables were built for, not with the previous shape
             update_shape();
-            upload_conv_tables();
+            if (dirty_conv || fft_len != tables_fft || n_out != tables_out) upload_conv_tables();
             d_lines.ensure(sizeof(LineGeom) * lines.size());
             hip_check(hipMemcpy(d_lines.p, lines.data(), sizeof(LineGeom) * lines.size(), hipMemcpyHostToDevice),
                       "hipMemcpy");

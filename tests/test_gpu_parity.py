@@ -257,3 +257,32 @@ def test_tiled_matches_direct_full_volume():
     err = rel_l2(res["tiled"], res["direct"])
     print(f"[result] full volume tiled vs direct rel_l2={err:.3e}", flush=True)
     assert err < 1e-6
+
+
+def test_scan_length_changes_between_frames():
+    """One simulator whose scan length, and so its FFT length (2048 - 16384) and
+    output size, changes between frames, with the output shape queried before each
+    frame (simulate_frame does): every frame equals a fresh simulator's and the
+    oracle's.  The convolution tables must follow the new frame shape."""
+    from openbcsim_amd.pyrfsim import RfSimulator
+    g = RfSimulator("gpu")
+    first = True
+    for length in (0.028, 0.12, 0.15, 0.06, 0.15, 0.028, 0.12):
+        wl = W.fixed_sector_2d(6000, 12, seed=91, line_length=length, decimation=2)
+        if first:
+            wl.apply(g)
+            first = False
+        else:
+            o, d, l, t, ln = wl.scan
+            g.set_scan_sequence(o, d, ln, l, t, False)
+        iq = g.simulate_frame()
+        fresh = gpu_sim()
+        wl.apply(fresh)
+        ref = fresh.simulate_frame()
+        fresh.close()
+        e = rel_l2(iq, ref)
+        print(f"[result] line length {length} N_fft={wl.fft_len()} vs fresh simulator {e:.2e}", flush=True)
+        assert iq.shape == ref.shape and e < 1e-6
+    o = Oracle()
+    wl.apply(o)
+    assert rel_l2(iq, o.simulate_frame()) <= TOL
