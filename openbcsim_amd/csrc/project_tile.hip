@@ -126,9 +126,23 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     const int b0 = window * g.window_bins;
     const int nb = min(g.window_bins, c.num_rf - b0);
     const int stride = g.window_bins * E;  // LDS floats per line
-    float* bins = lds;
-    int* list_chunk = reinterpret_cast<int*>(lds + (size_t)g.tile_lines * stride);
+    // LDS: the tile's line geometry and spline basis (read for every hit; kept here
+    // rather than re-read from global memory), then the bins, then the chunk list
+    LineGeom* tl = reinterpret_cast<LineGeom*>(lds);
+    SplineLineBasis* tb = reinterpret_cast<SplineLineBasis*>(tl + g.tile_lines);
+    float* bins = reinterpret_cast<float*>(tb + g.tile_lines);
+    int* list_chunk = reinterpret_cast<int*>(bins + (size_t)g.tile_lines * stride);
     unsigned* list_mask = reinterpret_cast<unsigned*>(list_chunk + g.list_cap);
+    {
+        const float4* src = reinterpret_cast<const float4*>(c.lines + line0);
+        float4* dst = reinterpret_cast<float4*>(tl);
+        for (int i = threadIdx.x; i < nl * 4; i += blockDim.x) dst[i] = src[i];
+        if (SPLINE) {
+            const float4* bs = reinterpret_cast<const float4*>(ss.basis + line0);
+            float4* bd = reinterpret_cast<float4*>(tb);
+            for (int i = threadIdx.x; i < nl * 2; i += blockDim.x) bd[i] = bs[i];
+        }
+    }
     zero_lds(bins, nl * stride);
     if (threadIdx.x == 0) list_n = 0;
 
@@ -137,7 +151,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     const float r_hi = fminf(cp.r_hi, ((float)(b0 + nb) - 0.5f) * cp.r_per_bin);
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
     const int nwaves = blockDim.x >> 6;
     const int cbeg = slice * g.chunks_per_slice;
     const int cend = min(nchunks, cbeg + g.chunks_per_slice);
@@ -153,10 +167,10 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
         const int ci = next + (int)threadIdx.x;
         if ((int)threadIdx.x < step && ci < cend) {
             const ChunkBox B = boxes[ci];
-            if (!(SHARED && ARC) || chunk_in_window(r_lo, r_hi, c.lines[line0], B)) {
+            if (!(SHARED && ARC) || chunk_in_window(r_lo, r_hi, tl[0], B)) {
                 unsigned mask = 0;
                 for (int k = 0; k < nl; k++)
-                    if (chunk_hits<ARC, LUT>(cp, r_lo, r_hi, c.lines[line0 + k], B)) mask |= 1u << k;
+                    if (chunk_hits<ARC, LUT>(cp, r_lo, r_hi, tl[k], B)) mask |= 1u << k;
                 if (mask) {
                     const int slot = atomicAdd(&list_n, 1);
                     list_chunk[slot] = ci;
@@ -171,8 +185,10 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
         if (next < cend && n <= g.list_cap - step) continue;
         // (B) project the listed chunks, one wave per chunk
         for (int j = wave; j < n; j += nwaves) {
-            const int ci = list_chunk[j];
-            unsigned mask = list_mask[j];
+            // one chunk per wave: its index and hit mask are wave-uniform, so the
+            // loop over hit lines is scalar and the line data is a broadcast LDS read
+            const int ci = __builtin_amdgcn_readfirstlane(list_chunk[j]);
+            unsigned mask = __builtin_amdgcn_readfirstlane(list_mask[j]);
             const int s = ci * kChunk + lane;
             if (s >= N) continue;
             if (SPLINE) {
@@ -181,7 +197,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 while (mask) {
                     const int k = __builtin_ctz(mask);
                     mask &= mask - 1;
-                    const SplineLineBasis& Bk = ss.basis[line0 + k];
+                    const SplineLineBasis& Bk = tb[k];
                     // CpuAlgorithm.cpp:137-140: float sum from zero, in control-point order
                     float px = 0.0f, py = 0.0f, pz = 0.0f;
                     for (int i = 0; i < Bk.count; i++) {
@@ -190,14 +206,14 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                         py = py + ss.cy[idx] * Bk.b[i];
                         pz = pz + ss.cz[idx] * Bk.b[i];
                     }
-                    const Local q = local_coords<ARC>(c.lines[line0 + k], px, py, pz);
+                    const Local q = local_coords<ARC>(tl[k], px, py, pz);
                     double ti;
                     const int bin = radial_bin(c, q.r, ti);
                     emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, amp);
                 }
             } else if (SHARED && ARC) {
                 const float4 p = fs.pts[s];
-                const LineGeom& L0 = c.lines[line0];
+                const LineGeom& L0 = tl[0];
                 const float dx = p.x - L0.ox, dy = p.y - L0.oy, dz = p.z - L0.oz;
                 const float absd = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
                 double ti_pos;
@@ -209,17 +225,16 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 while (mask) {
                     const int k = __builtin_ctz(mask);
                     mask &= mask - 1;
-                    const LineGeom& L = c.lines[line0 + k];
+                    const LineGeom& L = tl[k];
+                    // all three coordinates before branching: one broadcast read of the line
                     const float rp = dx * L.rx + dy * L.ry + dz * L.rz;
+                    const float l = dx * L.lx + dy * L.ly + dz * L.lz;
+                    const float e = dx * L.ex + dy * L.ey + dz * L.ez;
                     if (!signbit(rp)) {
                         if (!in_pos) continue;
-                        const float l = dx * L.lx + dy * L.ly + dz * L.lz;
-                        const float e = dx * L.ex + dy * L.ey + dz * L.ez;
                         const float w = (LUT ? lut_weight_le(c.prof, R, l, e) : gauss_weight(c.prof, l, e)) * p.w;
                         add_bin<PHASE>(c, bins + k * stride, lb_pos, bin_pos, ti_pos, w);
                     } else {
-                        const float l = dx * L.lx + dy * L.ly + dz * L.lz;
-                        const float e = dx * L.ex + dy * L.ey + dz * L.ez;
                         double ti;
                         const int bin = radial_bin(c, -absd, ti);
                         emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w);
@@ -230,7 +245,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 while (mask) {
                     const int k = __builtin_ctz(mask);
                     mask &= mask - 1;
-                    const Local q = local_coords<ARC>(c.lines[line0 + k], p.x, p.y, p.z);
+                    const Local q = local_coords<ARC>(tl[k], p.x, p.y, p.z);
                     double ti;
                     const int bin = radial_bin(c, q.r, ti);
                     emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, p.w);
@@ -260,7 +275,7 @@ template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
 __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullParams cp, FixedSet fs, SplineSet ss,
                                                             const ChunkBox* __restrict__ boxes, int nchunks,
                                                             float* __restrict__ out, TileLaunch g) {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ int list_n;
     tile_block<ARC, PHASE, LUT, SPLINE, SHARED>(c, cp, fs, ss, boxes, nchunks, out, g, blockIdx.x, blockIdx.y, lds,
                                                 list_n);
@@ -285,7 +300,7 @@ __device__ __forceinline__ void tile_block_job(const ProjJob& J, int wg, int sli
 __global__ __launch_bounds__(1024) void project_batch_kernel(const ProjJob* __restrict__ jobs,
                                                              const int2* __restrict__ items, int n_items,
                                                              int* __restrict__ counter) {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ int list_n;
     __shared__ int item_s;
     for (;;) {
@@ -332,7 +347,7 @@ hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_it
 }
 
 size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap) {
-    return (size_t)tile_lines * (size_t)window_bins * (complex_bins ? 8 : 4) + (size_t)list_cap * 8;
+    return (size_t)tile_lines * ((size_t)window_bins * (complex_bins ? 8 : 4) + kTileLineLds) + (size_t)list_cap * 8;
 }
 
 template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>

@@ -116,6 +116,7 @@ struct ConvParams {
 // ---- culled tile projection (project_tile.hip) ----
 
 constexpr int kChunk = 64;      // scatterers per chunk: one per lane of a wave
+constexpr int kTileLineLds = 96; // LDS bytes per tile line beside its bins: LineGeom + SplineLineBasis
 constexpr int kMaxListCap = 2048;  // chunk list entries held in LDS between drains
 
 // Axis-aligned box around the (finite) positions a chunk can take: the points

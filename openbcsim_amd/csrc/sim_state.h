@@ -992,13 +992,14 @@ struct bcsim_sim {
         const size_t bin_bytes = phase_delay ? 8 : 4;
         const size_t avail = (size_t)kLdsBytes - 64 * 8 - 256;  // keep room for a 64-entry list
         // radial windows: one by default (measured fastest on the workloads of the tests
-        // and bench); more if one line's bins do not fit in LDS
+        // and bench); more if one line's bins (and its geometry, kTileLineLds) do not fit
         int K = tune.windows > 0 ? tune.windows : 1;
         K = std::max(1, std::min(K, num_rf));
-        while ((size_t)((num_rf + K - 1) / K) * bin_bytes > avail) K++;
+        while ((size_t)((num_rf + K - 1) / K) * bin_bytes + kTileLineLds > avail) K++;
         const int W = (num_rf + K - 1) / K;
         K = (num_rf + W - 1) / W;
-        const int tmax = (int)std::max<size_t>(1, std::min<size_t>(32, avail / ((size_t)W * bin_bytes)));
+        const size_t per_line = (size_t)W * bin_bytes + kTileLineLds;
+        const int tmax = (int)std::max<size_t>(1, std::min<size_t>(32, avail / per_line));
         int T;
         if (tune.lines > 0) {
             T = std::min(tune.lines, tmax);
@@ -1020,7 +1021,7 @@ struct bcsim_sim {
         cfg.block = tune.block > 0 ? tune.block : 1024;
         cfg.atomic = cfg.slices > 1 ? 1 : 0;
         // the rest of the LDS holds the chunk list
-        const size_t left = (size_t)kLdsBytes - 256 - (size_t)T * W * bin_bytes;
+        const size_t left = (size_t)kLdsBytes - 256 - (size_t)T * per_line;
         cfg.list_cap = (int)std::min<size_t>(kMaxListCap, (left / 8) / 64 * 64);
         return cfg;
     }

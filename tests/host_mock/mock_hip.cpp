@@ -439,7 +439,7 @@ void check_conv(const ConvParams& p, const char* op) {
 }  // namespace
 
 size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap) {
-    return (size_t)tile_lines * (size_t)window_bins * (complex_bins ? 8 : 4) + (size_t)list_cap * 8;
+    return (size_t)tile_lines * ((size_t)window_bins * (complex_bins ? 8 : 4) + kTileLineLds) + (size_t)list_cap * 8;
 }
 size_t convolve_batch_lds_bytes() { return ((size_t)kMaxFftLen + 2 * 128) * sizeof(float2); }
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
