@@ -36,6 +36,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_EVAL = {"fixed": 16, "spline": 52}
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, the scales a 256 MiB device copy reads back as), for the committed kernels;
+# profiles/project_tile_volume_lut_1M_4096_pmc.md gives the commands and counts.
+MEASURED_TRAFFIC = {"volume_lut": {"bytes": 3.704e6 * 1024 * 2.0 + 1.574e5 * 1024,
+                                   "source": "profiles/project_tile_volume_lut_1M_4096_pmc.md"}}
 
 
 def build_workload(name: str):
@@ -383,9 +388,15 @@ def main():
     if proj_ms:
         algo_bytes = BYTES_PER_EVAL[kind] * evals
         achieved = algo_bytes / (proj_ms * 1e-3) / 1e9
+        measured = MEASURED_TRAFFIC.get(wl.name) if not args.tile else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": pkey.replace("_ms", ""), "kernel_ms": round(proj_ms, 4)}
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": round(measured["bytes"]) if measured else None,
+                    "kernel": pkey.replace("_ms", ""), "kernel_ms": round(proj_ms, 4),
+                    "algorithmic_bytes": algo_bytes}
+        if measured:
+            roofline["traffic_source"] = measured["source"]
+            roofline["traffic_gbs"] = round(measured["bytes"] / (proj_ms * 1e-3) / 1e9, 1)
     line = {
         "metric": "IQ lines/s & scatterer-evals/s at 1M scatterers; serving p50/p99 frame latency",
         "value": round(L / (ms * 1e-3), 1),
