@@ -262,8 +262,26 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int N, int Ls, const 
         if (j < m) {
             const int k = j & (Ls - 1);
             const int step = k * (N / (Ls * R));
+            if (R == 8) {
+                // w^1, w^2, w^4 from the table; w^3, w^5, w^6, w^7 as products of two of
+                // them: 6 table reads per butterfly instead of 14
+                if (Ls > 1) {
+                    const float2 w1 = tw.template get<INV>(step);
+                    const float2 w2 = tw.template get<INV>(2 * step);
+                    const float2 w4 = tw.template get<INV>(4 * step);
+                    const float2 w3 = cmul(w1, w2);
+                    x[it][1] = cmul(x[it][1], w1);
+                    x[it][2] = cmul(x[it][2], w2);
+                    x[it][3] = cmul(x[it][3], w3);
+                    x[it][4] = cmul(x[it][4], w4);
+                    x[it][5] = cmul(x[it][5], cmul(w1, w4));
+                    x[it][6] = cmul(x[it][6], cmul(w2, w4));
+                    x[it][7] = cmul(x[it][7], cmul(w3, w4));
+                }
+            } else if (Ls > 1) {  // Ls == 1: every twiddle is 1
 #pragma unroll
-            for (int q = 1; q < R; q++) x[it][q] = cmul(x[it][q], tw.template get<INV>(q * step));
+                for (int q = 1; q < R; q++) x[it][q] = cmul(x[it][q], tw.template get<INV>(q * step));
+            }
             float2 y[R];
             if (R == 2) {
                 y[0] = cadd(x[it][0], x[it][1]);
