@@ -264,27 +264,45 @@ def run_serving(seconds: float = 10.0, rate: float = 200.0, scale: float = 1.0, 
             "saturated": sat}
 
 
-def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300):
+def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300, clients: int = 8):
     """Throughput with every request queued at once: the server (continuous
-    batching) against per-call loops over the same requests -- one simulator per
-    phantom, per request set_scan_sequence (the geometry, every timestamp t) and a
+    batching; `clients` threads wait for the requests, as concurrent clients do)
+    against per-call loops over the same requests -- one simulator per phantom,
+    per request set_scan_sequence (the geometry, every timestamp t) and a
     synchronous simulate_lines into host memory, the reference's call pattern
     (MainWindow.cpp:666-695) -- with the reference's projection method (one
-    thread per scatterer-line pair, "direct") and with the culled tile kernel."""
+    thread per scatterer-line pair, "direct") and with the culled tile kernel.
+    Both forms write into output buffers allocated and touched beforehand."""
+    import threading
     reqs = [(*pairs[int(rng.integers(len(pairs)))], float(rng.uniform(0, 1))) for _ in range(n)]
     lines = sum(geometries[g][0].shape[0] for _, g, _ in reqs)
+    outs = []
+    for p, g, _ in reqs:
+        nl, ns = srv.output_shape(p, g)
+        buf = np.empty((nl, ns), np.complex64)
+        buf.fill(0)
+        outs.append(buf)
 
     def burst():
         srv.set_parameter("reset_stats", 1)
         t0 = time.perf_counter()
-        handles = [srv.submit(p, g, t) for p, g, t in reqs]
-        for h in handles:
-            srv.wait(h)
+        handles = [srv.submit(p, g, t, out=o) for (p, g, t), o in zip(reqs, outs)]
+
+        def client(k):
+            for h in handles[k::clients]:
+                srv.wait(h)
+
+        threads = [threading.Thread(target=client, args=(k,)) for k in range(clients)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
         return time.perf_counter() - t0
 
     cold = burst()    # the first burst of this depth also sizes the server's pinned staging pool
     served = burst()
     jobs = srv.stats("frame_jobs")
+    shapes = {}
 
     def loop(mode):
         for s in sims:
@@ -293,15 +311,18 @@ def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300):
         for p, g, t in reqs:
             o, d, l, length = geometries[g]
             sims[p].set_scan_sequence(o, d, length, l, np.full(o.shape[0], t, np.float32), True)
-            sims[p].simulate_frame()
+            key = (p, g)
+            if key not in shapes:
+                shapes[key] = np.zeros(sims[p].output_shape(), np.complex64)
+            sims[p].simulate_frame(out=shapes[key])
         return time.perf_counter() - t0
 
-    loop("tiled")  # set-up of every (phantom, geometry) shape
+    loop("tiled")  # set-up of every (phantom, geometry) shape and output buffer
     tiled = loop("tiled")
     direct = loop("direct")
     for s in sims:
         s.set_parameter("projection_mode", "tiled")
-    return {"requests": n, "lines": lines, "served_lines_per_s": round(lines / served, 1),
+    return {"requests": n, "lines": lines, "clients": clients, "served_lines_per_s": round(lines / served, 1),
             "first_burst_lines_per_s": round(lines / cold, 1),
             "frames": int(len(jobs)), "mean_requests_per_frame": round(float(np.mean(jobs)), 2),
             "per_call_lines_per_s": round(lines / tiled, 1),

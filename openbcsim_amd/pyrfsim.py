@@ -142,10 +142,16 @@ class RfSimulator:
         check(self._lib.bcsim_get_output_shape(self._h, C.byref(L), C.byref(S)))
         return L.value, S.value
 
-    def simulate_frame(self) -> np.ndarray:
-        """IQ frame as complex64 [line][sample]."""
+    def simulate_frame(self, out: np.ndarray = None) -> np.ndarray:
+        """IQ frame as complex64 [line][sample] (into `out` when given: C-contiguous
+        complex64 of at least lines*samples elements, returned reshaped)."""
         L, S = self.output_shape()
-        out = np.empty((L, S), np.complex64)
+        if out is None:
+            out = np.empty((L, S), np.complex64)
+        else:
+            if out.dtype != np.complex64 or not out.flags.c_contiguous or out.size < L * S:
+                raise ValueError("out must be a C-contiguous complex64 array of at least lines*samples elements")
+            out = out.reshape(-1)[: L * S].reshape(L, S)
         nl, ns = C.c_int(), C.c_int()
         check(self._lib.bcsim_simulate_lines(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), out.size,
                                              C.byref(nl), C.byref(ns)))
