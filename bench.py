@@ -19,6 +19,11 @@ once per line in the per-line streaming model) times L*N evaluations per launch.
 cpu_baseline: the reference's own CPU algorithm (CpuAlgorithm, compiled from the
 reference sources into oracle/_ref) on a bounded sample of the same workload --
 the first lines of the scan against all scatterers -- with all host cores.
+
+Further legs in the same JSON line (each can be skipped): `configs` (the 2D sector,
+the spline M-mode and the volume, phase delay off and on), `batched_64_jobs` (64
+uneven jobs per call vs batched) and `serving` (Poisson arrivals against 16 resident
+phantoms: p50/p99 latency; saturated throughput against per-call loops).
 """
 from __future__ import annotations
 
@@ -101,6 +106,38 @@ def run_gpu(wl, steps: int, warmup: int, tile=None):
     sim.close()
     out.close()
     return L, S, ms_per_step, {k: float(np.mean(v)) for k, v in kern.items() if v}, host_ms
+
+
+def run_configs(steps: int = 5):
+    """Every single-simulator config with phase delay off and on (device-resident IQ,
+    frame time over `steps` frames after one warm-up): the 2D sector (100k points,
+    128 lines), the spline M-mode (200k cubic splines, 256 lines, one time per
+    line) and the 1M-point volume."""
+    from openbcsim_amd import workloads as W
+    from openbcsim_amd.hip_rt import DeviceBuffer
+    from openbcsim_amd.pyrfsim import RfSimulator
+    out = {}
+    for name, make in (("fixed_sector_2d", W.fixed_sector_2d), ("spline_mmode", W.spline_mmode),
+                       ("volume_lut", W.volume_lut)):
+        for phase in (False, True):
+            wl = make(phase_delay=phase)
+            sim = RfSimulator("gpu")
+            wl.apply(sim)
+            L, S = sim.output_shape()
+            buf = DeviceBuffer(L * S * 8)
+            sim.simulate_lines_device(buf.address, L * S)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                sim.simulate_lines_device(buf.address, L * S)
+            sim.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+            out[f"{name}{'_phase' if phase else ''}"] = {
+                "lines": L, "scatterers": wl.num_scatterers, "ms_per_frame": round(dt * 1e3, 4),
+                "iq_lines_per_s": round(L / dt, 1), "scatterer_evals_per_s": round(L * wl.num_scatterers / dt, 1)}
+            sim.close()
+            buf.close()
+    return out
 
 
 def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
@@ -377,6 +414,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched-vs-per-call leg")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scatterer-count scale of the 64 jobs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the per-config (phase off/on) leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the timed serving run")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--tile", default=None, help="projection launch shape 'windows,lines,block' (default: automatic)")
@@ -439,6 +477,8 @@ def main():
         "pcie_inclusive_lines_per_s": round(L / (host_ms * 1e-3), 1),
         "roofline": roofline,
     }
+    if not args.no_configs:
+        line["configs"] = run_configs()
     if not args.no_batch:
         line["batched_64_jobs"] = run_batched(64, args.batch_scale)
     if not args.no_serve:
