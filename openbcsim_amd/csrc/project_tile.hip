@@ -101,6 +101,19 @@ __device__ __forceinline__ void add_bin(const ProjCommon& c, float* lbins, int l
     }
 }
 
+// add_bin with the sub-sample rotation (cos, sin) of the phase already computed
+// (the same for every line when the lines share an origin).
+template <bool PHASE>
+__device__ __forceinline__ void add_bin_rot(float* lbins, int lb, float w, float co, float s) {
+    if (w == 0.0f) return;
+    if (PHASE) {
+        atomicAdd(&lbins[2 * lb], w * co);
+        atomicAdd(&lbins[2 * lb + 1], w * s);
+    } else {
+        atomicAdd(&lbins[lb], w);
+    }
+}
+
 template <bool PHASE, bool LUT>
 __device__ __forceinline__ void emit(const ProjCommon& c, float* lbins, int b0, int nb, float r, float l, float e,
                                      int bin, double ti, float amp) {
@@ -222,6 +235,9 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 const bool in_pos = bin_pos >= 0 && lb_pos >= 0 && lb_pos < nb;
                 LutRadial R{};
                 if (LUT) R = lut_radial(c.prof, absd);
+                // phase of the positive-side bin: one sincos per scatterer, not per line
+                float co_pos = 1.0f, s_pos = 0.0f;
+                if (PHASE && in_pos) sincosf((float)((double)bin_pos - ti_pos) * c.phase_per_sample, &s_pos, &co_pos);
                 while (mask) {
                     const int k = __builtin_ctz(mask);
                     mask &= mask - 1;
@@ -233,7 +249,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     if (!signbit(rp)) {
                         if (!in_pos) continue;
                         const float w = (LUT ? lut_weight_le(c.prof, R, l, e) : gauss_weight(c.prof, l, e)) * p.w;
-                        add_bin<PHASE>(c, bins + k * stride, lb_pos, bin_pos, ti_pos, w);
+                        add_bin_rot<PHASE>(bins + k * stride, lb_pos, w, co_pos, s_pos);
                     } else {
                         double ti;
                         const int bin = radial_bin(c, -absd, ti);
