@@ -34,15 +34,17 @@ __device__ __forceinline__ float gauss_weight(const ProfileParams& P, float l, f
 
 // LUTBeamProfile::sampleProfile (BeamProfile.cpp:79-130): truncating indices, zero
 // whenever a neighbour falls outside the table, trilinear in r, then l, then e.
-// The table is read as quads: quad[(r*nl + l)*ne + e] holds the samples at
-// (l, e), (l, e+1), (l+1, e), (l+1, e+1) of radial slice r, so the eight corners
-// are two 16-byte loads.  Upper neighbour = lower + 1, except that a coordinate
+// The table is read as quad pairs: quad[2*((r*nl + l)*ne + e) + s] holds the
+// samples at (l, e), (l, e+1), (l+1, e), (l+1, e+1) of radial slice r + s, so the
+// eight corners are two 16-byte loads from one 32-byte record (one cache line).
+// Upper neighbour = lower + 1, except that a coordinate
 // in (-1, 0) truncates to 0 for both (the reference's int(t) and int(t + 1)).
 // The radial part depends on r only, so a caller projecting one point onto
 // several lines from a common origin computes it once (lut_radial).  The weight
 // feeds no discontinuity, so its interpolation uses fused lerps a + f*(b - a).
 struct LutRadial {
-    int off0, off1;  // quad offsets of radial slices r0 and r1
+    int off;   // cell offset of radial slice r0
+    bool low;  // r in (-1, 0) steps: the upper radial neighbour is r0 itself
     float fr;
     bool ok;
 };
@@ -52,9 +54,8 @@ __device__ __forceinline__ LutRadial lut_radial(const ProfileParams& P, float r)
     const float tr = (r - P.r_min) * P.inv_dr;
     R.ok = tr > -1.0f && tr < (float)(P.nr - 1);
     const int r0 = R.ok ? (int)tr : 0;
-    const int slice = P.nl * P.ne;
-    R.off0 = r0 * slice;
-    R.off1 = (tr < 0.0f ? r0 : r0 + 1) * slice;
+    R.off = r0 * (P.nl * P.ne);
+    R.low = tr < 0.0f;
     R.fr = tr - (float)r0;
     return R;
 }
@@ -70,9 +71,10 @@ __device__ __forceinline__ float lut_weight_le(const ProfileParams& P, const Lut
     if (!(R.ok && tl > -1.0f && tl < (float)(P.nl - 1) && te > -1.0f && te < (float)(P.ne - 1))) return 0.0f;
     const int l0 = (int)tl, e0 = (int)te;
     const float fl = tl - (float)l0, fe = te - (float)e0;
-    const int row = l0 * P.ne + e0;
-    float4 a = P.quad[R.off0 + row];
-    float4 b = P.quad[R.off1 + row];
+    const float4* cell = P.quad + 2 * (R.off + l0 * P.ne + e0);
+    float4 a = cell[0];
+    float4 b = cell[1];
+    if (R.low) b = a;
     if (te < 0.0f || tl < 0.0f) {
         if (te < 0.0f) {
             a.y = a.x; a.w = a.z;

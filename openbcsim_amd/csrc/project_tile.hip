@@ -114,6 +114,15 @@ __device__ __forceinline__ void add_bin_rot(float* lbins, int lb, float w, float
     }
 }
 
+// Directions of one line, held in registers.
+struct LineDirs {
+    float rx, ry, rz, lx, ly, lz, ex, ey, ez;
+};
+
+__device__ __forceinline__ LineDirs line_dirs(const LineGeom& L) {
+    return LineDirs{L.rx, L.ry, L.rz, L.lx, L.ly, L.lz, L.ex, L.ey, L.ez};
+}
+
 template <bool PHASE, bool LUT>
 __device__ __forceinline__ void emit(const ProjCommon& c, float* lbins, int b0, int nb, float r, float l, float e,
                                      int bin, double ti, float amp) {
@@ -174,7 +183,23 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     // might not fit, so a workgroup synchronises about twice per step and the
     // drains hand its waves long lists.
     const int step = min((int)blockDim.x, g.list_cap);
+    int hits = 0;  // (chunk, line) pairs this thread listed
     __syncthreads();
+    // Tiles of up to kRegLines lines sharing an origin keep the origin and line
+    // directions in registers for the whole kernel: the per-hit loop then reads
+    // no LDS besides its additions, so no wait on the LDS queue (which holds the
+    // additions of earlier hits) sits between a hit's table gathers and the next.
+    constexpr int kRegLines = 4;
+    const bool reg_lines = SHARED && ARC && !SPLINE && nl <= kRegLines;
+    LineDirs G[kRegLines];
+    float org_x = 0.0f, org_y = 0.0f, org_z = 0.0f;
+    if (SHARED && ARC && !SPLINE) {
+#pragma unroll
+        for (int k = 0; k < kRegLines; k++) G[k] = line_dirs(tl[k < nl ? k : 0]);
+        org_x = tl[0].ox;
+        org_y = tl[0].oy;
+        org_z = tl[0].oz;
+    }
     for (int next = cbeg; next < cend;) {
         // (A) cull the next `step` chunks
         const int ci = next + (int)threadIdx.x;
@@ -188,6 +213,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     const int slot = atomicAdd(&list_n, 1);
                     list_chunk[slot] = ci;
                     list_mask[slot] = mask;
+                    hits += __popc(mask);
                 }
             }
         }
@@ -226,8 +252,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 }
             } else if (SHARED && ARC) {
                 const float4 p = fs.pts[s];
-                const LineGeom& L0 = tl[0];
-                const float dx = p.x - L0.ox, dy = p.y - L0.oy, dz = p.z - L0.oz;
+                const float dx = p.x - org_x, dy = p.y - org_y, dz = p.z - org_z;
                 const float absd = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
                 double ti_pos;
                 const int bin_pos = radial_bin(c, absd, ti_pos);
@@ -238,22 +263,36 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 // phase of the positive-side bin: one sincos per scatterer, not per line
                 float co_pos = 1.0f, s_pos = 0.0f;
                 if (PHASE && in_pos) sincosf((float)((double)bin_pos - ti_pos) * c.phase_per_sample, &s_pos, &co_pos);
-                while (mask) {
-                    const int k = __builtin_ctz(mask);
-                    mask &= mask - 1;
-                    const LineGeom& L = tl[k];
-                    // all three coordinates before branching: one broadcast read of the line
-                    const float rp = dx * L.rx + dy * L.ry + dz * L.rz;
-                    const float l = dx * L.lx + dy * L.ly + dz * L.lz;
-                    const float e = dx * L.ex + dy * L.ey + dz * L.ez;
+                auto hit = [&](int k, float rx, float ry, float rz, float lx, float ly, float lz, float ex, float ey,
+                               float ez) {
+                    const float rp = dx * rx + dy * ry + dz * rz;
+                    const float l = dx * lx + dy * ly + dz * lz;
+                    const float e = dx * ex + dy * ey + dz * ez;
                     if (!signbit(rp)) {
-                        if (!in_pos) continue;
+                        if (!in_pos) return;
                         const float w = (LUT ? lut_weight_le(c.prof, R, l, e) : gauss_weight(c.prof, l, e)) * p.w;
                         add_bin_rot<PHASE>(bins + k * stride, lb_pos, w, co_pos, s_pos);
                     } else {
                         double ti;
                         const int bin = radial_bin(c, -absd, ti);
                         emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w);
+                    }
+                };
+                if (reg_lines) {
+#pragma unroll
+                    for (int k = 0; k < kRegLines; k++) {
+                        if (mask & (1u << k)) {
+                            const LineDirs& D = G[k];
+                            hit(k, D.rx, D.ry, D.rz, D.lx, D.ly, D.lz, D.ex, D.ey, D.ez);
+                        }
+                    }
+                } else {
+                    while (mask) {
+                        const int k = __builtin_ctz(mask);
+                        mask &= mask - 1;
+                        // all three coordinates before branching: one broadcast read of the line
+                        const LineGeom& L = tl[k];
+                        hit(k, L.rx, L.ry, L.rz, L.lx, L.ly, L.lz, L.ex, L.ey, L.ez);
                     }
                 }
             } else {
@@ -271,6 +310,11 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
         __syncthreads();
         if (threadIdx.x == 0) list_n = 0;
         __syncthreads();
+    }
+    if (g.work) {
+        // the block's work, for the order of the next launch: one atomic per wave
+        for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
+        if (lane == 0 && hits) atomicAdd(&g.work[wg], hits);
     }
     __syncthreads();
     // flush: line k's window [b0, b0 + nb) of the global bins [line][num_rf]
@@ -293,8 +337,47 @@ __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullPa
                                                             float* __restrict__ out, TileLaunch g) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ int list_n;
-    tile_block<ARC, PHASE, LUT, SPLINE, SHARED>(c, cp, fs, ss, boxes, nchunks, out, g, blockIdx.x, blockIdx.y, lds,
-                                                list_n);
+    const int wg = g.order ? g.order[blockIdx.x] : (int)blockIdx.x;
+    tile_block<ARC, PHASE, LUT, SPLINE, SHARED>(c, cp, fs, ss, boxes, nchunks, out, g, wg, blockIdx.y, lds, list_n);
+}
+
+// Order of the next tile launch's blocks: sorted by the (chunk, line) pairs each
+// listed in the previous launch, heaviest first, so that the heaviest blocks
+// start in the first wave of workgroups and the launch does not end on a long
+// tail; the counts are then cleared for the next launch to fill.  Ties keep
+// the block order.  Bitonic sort of (work << 32 | ~block) keys in LDS.
+__global__ __launch_bounds__(1024) void order_tiles_kernel(int* __restrict__ work, int* __restrict__ order, int n) {
+    __shared__ unsigned long long key[kMaxOrderedTiles];
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+        key[i] = i < n ? ((unsigned long long)(unsigned)max(work[i], 0) << 32) | (0xffffffffu - (unsigned)i) : 0ull;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long a = key[i], b = key[ixj];
+                    if ((i & k) == 0 ? a < b : a > b) {
+                        key[i] = b;
+                        key[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        order[i] = (int)(0xffffffffu - (unsigned)(key[i] & 0xffffffffull));
+        work[i] = 0;
+    }
+}
+
+hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream) {
+    if (n <= 0 || n > kMaxOrderedTiles) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(order_tiles_kernel, dim3(1), dim3(1024), 0, stream, work, order, n);
+    return hipGetLastError();
 }
 
 // Persistent projection over many jobs (simulators x datasets): every

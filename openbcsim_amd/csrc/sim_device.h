@@ -48,7 +48,7 @@ struct ProfileParams {
     // Gaussian: exp(-(l*l*inv_tsl + e*e*inv_tse)), inv_ts* = 1/(2 sigma^2)
     float inv_tsl, inv_tse;
     // LUT (BeamProfile.cpp:79-130): t = (x - x_min) * inv_dx, trilinear, zero outside;
-    // read as quads (see lut_weight): quad[(r*nl + l)*ne + e]
+    // read as quad pairs (see lut_weight): quad[2*((r*nl + l)*ne + e) + s] holds slice r + s
     const float4* quad;
     int nr, nl, ne;
     float r_min, l_min, e_min;
@@ -147,7 +147,12 @@ struct TileLaunch {
     int block;                // threads per workgroup (multiple of 64)
     int atomic;               // flush with global atomics
     int list_cap;             // chunk list entries in LDS (multiple of 64)
+    const int* order;         // workgroup w runs block order[w] (heaviest first), or null: block w
+    int* work;                // per-block hit counts added up by the launch, or null
 };
+
+// Blocks one order_tiles launch can sort (64 KiB of LDS keys).
+constexpr int kMaxOrderedTiles = 8192;
 
 // One projection job of a batch: a (simulator, dataset) pair with everything its
 // tile kernel would receive as arguments.
@@ -192,6 +197,9 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams& cp, const 
                                bool arc, bool phase, bool lut, bool shared_origin, const TileLaunch& cfg,
                                hipStream_t stream);
 size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap);
+// Sorts blocks 0..n-1 (n <= kMaxOrderedTiles) by work[] descending into order[]
+// and clears work[] for the next tile launch to fill.
+hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream);
 
 // Batched frames (one launch each for all jobs of a batch): persistent
 // workgroups take items from `counter` (reset by launch_fill_segments).

@@ -247,6 +247,11 @@ struct bcsim_sim {
     // launch shape of the tile kernel set by the user; 0 = tuned or heuristic
     TileTune tune;
     bool autotune = false;  // time candidate launch shapes once per shape class
+    // Tile launches of more than one round of workgroups run their blocks
+    // heaviest first, by the work each block took in this simulator's previous
+    // frame (counted by the launch itself); the order never changes the sums.
+    bool tile_order = true;
+    std::vector<std::unique_ptr<DevBuf>> tile_order_bufs;  // per projection target: [work | order]
     int radix = 0;          // FFT pass radix set by the user (4 or 8); 0 = tuned or default
     std::string last_plan;  // launch shapes of the last frame, for get_parameter("launch_plan")
 
@@ -358,6 +363,8 @@ struct bcsim_sim {
             else fail(BCSIM_ERR_INVALID_ARGUMENT, "projection_mode must be 'tiled' or 'direct'");
         } else if (key == "autotune") {
             autotune = parse_bool(value);
+        } else if (key == "tile_order") {
+            tile_order = parse_bool(value);
         } else if (key == "fft_radix") {
             const int v = parse_int(value);
             if (v != 0 && v != 4 && v != 8) fail(BCSIM_ERR_INVALID_ARGUMENT, "fft_radix must be 4, 8 or 0 (automatic)");
@@ -401,6 +408,7 @@ struct bcsim_sim {
         if (key == "store_kernel_details") return store_details ? "on" : "off";
         if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
         if (key == "autotune") return autotune ? "on" : "off";
+        if (key == "tile_order") return tile_order ? "on" : "off";
         if (key == "fft_radix") return std::to_string(radix);
         if (key == "launch_plan") return last_plan;
         if (key == "autotune_report") return TuneCache::get().report();
@@ -663,20 +671,28 @@ struct bcsim_sim {
         if (!samples) fail(BCSIM_ERR_INVALID_ARGUMENT, "null LUT samples");
         use_device();
         quiesce();
-        // quads of the (l, e) neighbourhood, see lut_weight in proj_math.h
+        // quads of the (l, e) neighbourhood of slices r and r + 1, side by side (see
+        // lut_weight in proj_math.h); the last slice, never a lower neighbour, pairs
+        // with itself
         const size_t cells = (size_t)nr * nl * ne;
-        std::vector<float4> quad(cells);
+        std::vector<float4> quad(2 * cells);
         auto at = [&](int r, int l, int e) -> float {
             return (l < nl && e < ne) ? samples[((size_t)r * nl + l) * ne + e] : 0.0f;
         };
+        auto q = [&](int r, int l, int e) {
+            return make_float4(at(r, l, e), at(r, l, e + 1), at(r, l + 1, e), at(r, l + 1, e + 1));
+        };
         for (int r = 0; r < nr; r++)
             for (int l = 0; l < nl; l++)
-                for (int e = 0; e < ne; e++)
-                    quad[((size_t)r * nl + l) * ne + e] =
-                        make_float4(at(r, l, e), at(r, l, e + 1), at(r, l + 1, e), at(r, l + 1, e + 1));
+                for (int e = 0; e < ne; e++) {
+                    const size_t cell = ((size_t)r * nl + l) * ne + e;
+                    quad[2 * cell] = q(r, l, e);
+                    quad[2 * cell + 1] = q(r + 1 < nr ? r + 1 : r, l, e);
+                }
         auto table = std::make_shared<DevBuf>();
-        table->ensure(cells * sizeof(float4));
-        hip_check(hipMemcpy(table->p, quad.data(), cells * sizeof(float4), hipMemcpyHostToDevice), "hipMemcpy");
+        table->ensure(quad.size() * sizeof(float4));
+        hip_check(hipMemcpy(table->p, quad.data(), quad.size() * sizeof(float4), hipMemcpyHostToDevice),
+                  "hipMemcpy");
         lut = std::move(table);
         lut_nr = nr;
         lut_nl = nl;
@@ -1090,6 +1106,29 @@ struct bcsim_sim {
         hip_check(launch_render_spline(t.ss, B0, t.scan->rendered.as<float4>(), stream), "render kernel");
     }
 
+    // Blocks (tile, window) of one slice of a tile launch.
+    int tile_blocks(const TileLaunch& cfg) const {
+        return (((int)lines.size() + cfg.tile_lines - 1) / cfg.tile_lines) * cfg.windows;
+    }
+
+    // Fewer blocks than this start together on the 256 CUs: no order to choose.
+    static constexpr int kMinOrderedBlocks = 512;
+
+    // [work | order] for `blocks` blocks of projection target i (zeroed on the
+    // stream when the buffer is new; afterwards the order kernel clears the work
+    // counts every frame).
+    int* order_buffer(size_t i, int blocks) {
+        if (tile_order_bufs.size() <= i) tile_order_bufs.resize(i + 1);
+        auto& b = tile_order_bufs[i];
+        if (!b) b = std::make_unique<DevBuf>();
+        const size_t need = 2 * (size_t)blocks * sizeof(int);
+        if (b->bytes < need) {
+            b->ensure(need);
+            hip_check(launch_fill_zero(b->as<float>(), b->bytes / sizeof(float), stream), "fill kernel");
+        }
+        return b->as<int>();
+    }
+
     void launch_target(const ProjTarget& t, const TileLaunch& cfg, float* bins) {
         const bool is_spline = std::strcmp(t.kind, "spline") == 0;
         hip_check(launch_project_tile(t.c, cull_params(t.c), is_spline ? nullptr : &t.fs, is_spline ? &t.ss : nullptr,
@@ -1251,6 +1290,14 @@ struct bcsim_sim {
             } else {
                 TileLaunch cfg = plans[i];
                 cfg.atomic |= single ? 0 : 1;
+                const int blocks = tile_blocks(cfg);
+                if (tile_order && blocks >= kMinOrderedBlocks && blocks <= kMaxOrderedTiles) {
+                    int* wo = order_buffer(i, blocks);
+                    cfg.work = wo;
+                    cfg.order = wo + blocks;
+                    hip_check(launch_order_tiles(cfg.work, const_cast<int*>(cfg.order), blocks, stream),
+                              "tile order kernel");
+                }
                 launch_target(t, cfg, bins);
             }
             tm.stop();

@@ -10,6 +10,7 @@
 // so the launch-shape tuner picks random candidates.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -381,7 +382,7 @@ constexpr size_t kLds = 160 * 1024;
 
 void check_common(const ProjCommon& c, bool lut, const char* op) {
     check(c.lines, (size_t)c.num_lines * sizeof(LineGeom), "lines", op);
-    if (lut) check(c.prof.quad, (size_t)c.prof.nr * c.prof.nl * c.prof.ne * sizeof(float4), "lut", op);
+    if (lut) check(c.prof.quad, (size_t)c.prof.nr * c.prof.nl * c.prof.ne * 2 * sizeof(float4), "lut", op);
 }
 
 void check_spline(const SplineSet& s, int num_lines, bool basis, const char* op) {
@@ -419,6 +420,17 @@ void check_tile(const ProjCommon& c, const FixedSet* fs, const SplineSet* ss, co
     if (block < 64 || block > 1024 || block % 64) die(std::string(op) + ": block");
     const size_t lds = tile_lds_bytes(g.window_bins, phase, g.tile_lines, g.list_cap);
     if (lds + 8 > kLds) die(std::string(op) + ": LDS over 160 KiB");
+    const int blocks = ((c.num_lines + g.tile_lines - 1) / g.tile_lines) * g.windows;
+    if (g.work) check(g.work, (size_t)blocks * 4, "work", op);
+    if (g.order) {
+        check(g.order, (size_t)blocks * 4, "order", op);
+        std::vector<char> seen(blocks, 0);
+        for (int i = 0; i < blocks; i++) {
+            const int w = g.order[i];
+            if (w < 0 || w >= blocks || seen[w]) die(std::string(op) + ": order is not a permutation of the blocks");
+            seen[w] = 1;
+        }
+    }
 }
 
 void check_conv(const ConvParams& p, const char* op) {
@@ -500,6 +512,20 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams&, const Fix
                                  (const float*)bins, phase, lut, cfg, cfg.block, "project_tile");
                   },
                   "project_tile"});
+    return hipSuccess;
+}
+
+hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t s) {
+    if (n <= 0 || n > kMaxOrderedTiles) return hipErrorInvalidValue;
+    enqueue(s, Op{[=] {
+                      check(work, (size_t)n * 4, "work", "order_tiles");
+                      check(order, (size_t)n * 4, "order", "order_tiles");
+                      std::vector<int> idx(n);
+                      for (int i = 0; i < n; i++) idx[i] = i;
+                      std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return work[a] > work[b]; });
+                      for (int i = 0; i < n; i++) order[i] = idx[i], work[i] = 0;
+                  },
+                  "order_tiles"});
     return hipSuccess;
 }
 
