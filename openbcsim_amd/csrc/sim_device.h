@@ -100,11 +100,17 @@ struct ConvParams {
     int log2_len;
     int twiddle_bits;       // fine table holds 2^twiddle_bits entries
     const float2* twiddles; // coarse[2^(log2_len - twiddle_bits)] then fine[2^twiddle_bits]
+    // H[k] = mask[k] * FFT_N(excitation)[k] / N * exp(2 pi i k center / N) for k <= N/2,
+    // zero above: the spectrum of the delay-compensated output x[center + n]
     const float2* H;        // [fft_len]
     const float2* demod;    // [n_out]
     int center;
     int decimation;
     int n_out;
+    // the inverse transform is N / 2^fold_log2 points long (2^fold_log2 = the largest
+    // power of two dividing the decimation); output m is its sample out_stride * m
+    int fold_log2;
+    int out_stride;
     float2* iq;             // [L][n_out]
     // additive noise on the bins before convolution (CpuAlgorithm.cpp:340-347)
     float noise_amplitude;
@@ -112,6 +118,14 @@ struct ConvParams {
     uint32_t noise_counter;
     int radix;              // largest FFT pass radix: 4 or 8
 };
+
+// LDS points (float2) of one line's convolution buffer; its twiddle table follows
+// it.  Real bins without noise, folded (fold_log2 >= 1): the N/2-point transform
+// of the packed samples and the folded inverse both fit N/2 points.
+__host__ __device__ inline int conv_buffer_len(const ConvParams& p) {
+    const bool real_in = !p.complex_bins && !(p.noise_amplitude > 0.0f);
+    return (real_in && p.fold_log2 > 0) ? p.fft_len / 2 : p.fft_len;
+}
 
 // ---- culled tile projection (project_tile.hip) ----
 

@@ -191,9 +191,11 @@ struct Twiddle {
     const float2* fine;
     int bits;
     int mask;
-    // exp(-2 pi i idx / N) for 0 <= idx < N, conjugated for the inverse transform
+    int shift;  // the table holds exp(-2 pi i k / N); a transform of N >> shift points reads every 2^shift-th
+    // exp(-2 pi i idx / (N >> shift)) for 0 <= idx < N >> shift, conjugated for the inverse transform
     template <bool INV>
     __device__ __forceinline__ float2 get(int idx) const {
+        idx <<= shift;
         const float2 w = cmul(coarse[idx >> bits], fine[idx & mask]);
         return INV ? make_float2(w.x, -w.y) : w;
     }
@@ -237,12 +239,12 @@ __device__ __forceinline__ void dft8(const float2* x, float2* y) {
     y[7] = csub(e[3], o3);
 }
 
-// One Stockham autosort pass of radix R over N points held in LDS.  Ls is the
-// length of the sub-transforms already formed.  Every thread first reads all
-// its inputs, the workgroup synchronises, then every thread writes its outputs,
-// so one buffer suffices.
-template <int R, int NT, int NMAX, bool INV>
-__device__ __forceinline__ void stockham_pass(float2* buf, int N, int Ls, const Twiddle& tw) {
+// One Stockham autosort pass of radix R over N points: input i is src(i), output
+// i goes to dst(i, value).  Ls is the length of the sub-transforms already
+// formed.  With SYNC every thread reads all its inputs before the workgroup
+// synchronises and any thread writes (a pass in place over one LDS buffer).
+template <int R, int NT, int NMAX, bool INV, bool SYNC, class Src, class Dst>
+__device__ __forceinline__ void stockham_io(int N, int Ls, const Twiddle& tw, const Src& src, const Dst& dst) {
 #pragma clang fp contract(fast)
     constexpr int IT = (NMAX / R + NT - 1) / NT;
     const int m = N / R;
@@ -252,10 +254,10 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int N, int Ls, const 
         const int j = threadIdx.x + it * NT;
         if (j < m) {
 #pragma unroll
-            for (int q = 0; q < R; q++) x[it][q] = buf[j + q * m];
+            for (int q = 0; q < R; q++) x[it][q] = src(j + q * m);
         }
     }
-    __syncthreads();
+    if (SYNC) __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; it++) {
         const int j = threadIdx.x + it * NT;
@@ -293,40 +295,55 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int N, int Ls, const 
             }
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int s2 = 0; s2 < R; s2++) buf[base + s2 * Ls] = y[s2];
+            for (int s2 = 0; s2 < R; s2++) dst(base + s2 * Ls, y[s2]);
         }
     }
-    __syncthreads();
 }
 
-// Power-of-two FFT in LDS: radix-RMAX passes (4 or 8) after one smaller pass
-// that takes up the remainder of log2 N.
-template <int NT, int NMAX, bool INV, int RMAX>
-__device__ __forceinline__ void fft_lds(float2* buf, int N, int log2n, const Twiddle& tw) {
-    int Ls = 1;
+// Radix of the first pass: it takes up log2 n mod 3 (RMAX 8) or log2 n mod 2
+// (RMAX 4); every later pass has radix RMAX.
+template <int RMAX>
+__device__ __forceinline__ int first_radix(int log2n) {
     if (RMAX == 8) {
-        const int rem = log2n % 3;
-        if (rem == 1) {
-            stockham_pass<2, NT, NMAX, INV>(buf, N, Ls, tw);
-            Ls = 2;
-        } else if (rem == 2) {
-            stockham_pass<4, NT, NMAX, INV>(buf, N, Ls, tw);
-            Ls = 4;
-        }
-        while (Ls < N) {
-            stockham_pass<8, NT, NMAX, INV>(buf, N, Ls, tw);
-            Ls *= 8;
-        }
-    } else {
-        if (log2n & 1) {
-            stockham_pass<2, NT, NMAX, INV>(buf, N, Ls, tw);
-            Ls = 2;
-        }
-        while (Ls < N) {
-            stockham_pass<4, NT, NMAX, INV>(buf, N, Ls, tw);
-            Ls *= 4;
-        }
+        const int r = log2n % 3;
+        return r == 1 ? 2 : r == 2 ? 4 : 8;
     }
+    return (log2n & 1) ? 2 : 4;
+}
+
+template <int NT, int NMAX, bool INV, bool SYNC, class Src, class Dst>
+__device__ __forceinline__ void stockham_any(int r, int N, int Ls, const Twiddle& tw, const Src& src,
+                                             const Dst& dst) {
+    if (r == 2) stockham_io<2, NT, NMAX, INV, SYNC>(N, Ls, tw, src, dst);
+    else if (r == 4) stockham_io<4, NT, NMAX, INV, SYNC>(N, Ls, tw, src, dst);
+    else stockham_io<8, NT, NMAX, INV, SYNC>(N, Ls, tw, src, dst);
+}
+
+// Power-of-two FFT of n points through the LDS buffer `buf`: the first pass
+// reads its inputs from src0 (SYNC0: src0 reads buf, so the pass is in place),
+// the last pass writes its outputs to dstL (SYNCL: dstL writes buf), the passes
+// between work in place in buf.  Returns after a workgroup barrier when the
+// output is in buf (SYNCL); otherwise without one.
+template <int NT, int NMAX, bool INV, int RMAX, bool SYNC0, bool SYNCL, class Src, class Dst>
+__device__ __forceinline__ void fft_fused(float2* buf, int n, int log2n, const Twiddle& tw, const Src& src0,
+                                          const Dst& dstL) {
+    const auto lds_src = [buf](int i) { return buf[i]; };
+    const auto lds_dst = [buf](int i, float2 v) { buf[i] = v; };
+    const int r0 = first_radix<RMAX>(log2n);
+    if (n == r0) {  // a single pass
+        stockham_any<NT, NMAX, INV, SYNC0 || SYNCL>(r0, n, 1, tw, src0, dstL);
+        if (SYNCL) __syncthreads();
+        return;
+    }
+    stockham_any<NT, NMAX, INV, SYNC0>(r0, n, 1, tw, src0, lds_dst);
+    __syncthreads();
+    int Ls = r0;
+    for (; Ls * RMAX < n; Ls *= RMAX) {
+        stockham_io<RMAX, NT, NMAX, INV, true>(n, Ls, tw, lds_src, lds_dst);
+        __syncthreads();
+    }
+    stockham_io<RMAX, NT, NMAX, INV, SYNCL>(n, Ls, tw, lds_src, dstL);
+    if (SYNCL) __syncthreads();
 }
 
 // Counter-based normal variates for the noise path: a 32-bit integer hash
@@ -353,41 +370,107 @@ __device__ __forceinline__ float2 normal_pair(uint32_t seed, uint32_t counter, u
     return make_float2(rad * co, rad * s);
 }
 
-// One line: bins -> FFT -> x H -> IFFT -> extract, demodulate, decimate.
+// One line: bins -> spectrum -> x H -> inverse transform -> demodulate.
+//
+// Real bins (no noise): the N real samples b[n] are transformed as the N/2
+// complex points z[m] = b[2m] + i b[2m+1]; with Z = FFT_{N/2}(z) (Z[N/2] = Z[0]),
+// E[k] = (Z[k] + conj Z[N/2-k]) / 2 and O[k] = (Z[k] - conj Z[N/2-k]) / 2i are the
+// spectra of the even and odd samples and X[k] = E[k] + exp(-2 pi i k / N) O[k].
+// Only X[0..N/2] is formed: the Hilbert mask zeroes the rest.
+//
+// Output m is x[center + D m] (D the decimation).  H carries the shift by
+// `center`, so with F = 2^fold_log2 dividing D the samples x[center + F j] are
+// the (N/F)-point inverse DFT of the aliased spectrum V[j] = sum_q Y[j + q N/F]
+// (Y = X H), and output m is its sample (D/F) m: the inverse transform is N/F
+// points long instead of N.
+//
+// The bins are read by the first forward pass, Y (or V) is formed by the first
+// inverse pass as it reads its inputs, and the last inverse pass writes the
+// demodulated output: no pass of its own for any of the three.  The first
+// inverse pass reads every input it needs from the forward result before the
+// workgroup barrier and writes after it, so it runs in place.  The buffer is
+// conv_buffer_len(p) points (N/2 for real bins folded by F >= 2), the twiddle
+// table follows it.
 template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX>
 __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2* sbuf) {
+#pragma clang fp contract(fast)
     const int N = p.fft_len;
+    const int half = N / 2;
     float2* buf = sbuf;
-    float2* twc = sbuf + N;
+    float2* twc = sbuf + conv_buffer_len(p);
     const int n_coarse = N >> p.twiddle_bits;
     const int n_fine = 1 << p.twiddle_bits;
     float2* twf = twc + n_coarse;
     for (int i = threadIdx.x; i < n_coarse + n_fine; i += NT) twc[i] = p.twiddles[i];
-    // bins of this line, zero padded to N
-    if (COMPLEX_BINS) {
-        const float2* src = static_cast<const float2*>(p.bins) + (size_t)line * p.num_rf;
-        for (int n = threadIdx.x; n < N; n += NT) buf[n] = n < p.num_rf ? src[n] : make_float2(0.f, 0.f);
-    } else {
-        const float* src = static_cast<const float*>(p.bins) + (size_t)line * p.num_rf;
-        for (int n = threadIdx.x; n < N; n += NT) buf[n] = make_float2(n < p.num_rf ? src[n] : 0.f, 0.f);
-    }
-    if (p.noise_amplitude > 0.0f) {
-        for (int n = threadIdx.x; n < p.num_rf; n += NT) {
-            const float2 z = normal_pair(p.noise_seed, p.noise_counter, (uint32_t)line, (uint32_t)n);
-            buf[n] = make_float2(buf[n].x + p.noise_amplitude * z.x, buf[n].y + p.noise_amplitude * z.y);
-        }
-    }
     __syncthreads();
-    Twiddle tw{twc, twf, p.twiddle_bits, n_fine - 1};
-    fft_lds<NT, NMAX, false, RMAX>(buf, N, p.log2_len, tw);
-    // excitation spectrum with Hilbert mask and 1/N; zero above N/2 (the mask)
-    const int half = N / 2;
-    for (int k = threadIdx.x; k < N; k += NT) buf[k] = k <= half ? cmul(buf[k], p.H[k]) : make_float2(0.f, 0.f);
-    __syncthreads();
-    fft_lds<NT, NMAX, true, RMAX>(buf, N, p.log2_len, tw);
-    // delay compensation, demodulation, decimation
+    const Twiddle twN{twc, twf, p.twiddle_bits, n_fine - 1, 0};
+    const int M = N >> p.fold_log2;  // inverse transform length
+    const int log2m = p.log2_len - p.fold_log2;
+    const Twiddle twM{twc, twf, p.twiddle_bits, n_fine - 1, p.fold_log2};
+    const float2* __restrict__ H = p.H;
+    // demodulated output: sample out_stride * m of the inverse transform is output m
     float2* out = p.iq + (size_t)line * p.n_out;
-    for (int j = threadIdx.x; j < p.n_out; j += NT) out[j] = cmul(buf[p.center + j * p.decimation], p.demod[j]);
+    const int n_out = p.n_out, E = p.out_stride;
+    const float2* __restrict__ dem = p.demod;
+    const auto store = [=](int o, float2 v) {
+        if (E == 1) {
+            if (o < n_out) out[o] = cmul(v, dem[o]);
+        } else if (o % E == 0 && o / E < n_out) {
+            out[o / E] = cmul(v, dem[o / E]);
+        }
+    };
+    const bool real_in = !COMPLEX_BINS && !(p.noise_amplitude > 0.0f);
+    if (real_in) {
+        const float* src = static_cast<const float*>(p.bins) + (size_t)line * p.num_rf;
+        const int nrf = p.num_rf;
+        const auto load = [=](int m) {
+            const int n = 2 * m;
+            return make_float2(n < nrf ? src[n] : 0.f, n + 1 < nrf ? src[n + 1] : 0.f);
+        };
+        fft_fused<NT, NMAX, false, RMAX, false, true>(buf, half, p.log2_len - 1,
+                                                     Twiddle{twc, twf, p.twiddle_bits, n_fine - 1, 1}, load,
+                                                     [buf](int i, float2 v) { buf[i] = v; });
+        // X[k] H[k] from Z[k] and Z[N/2 - k]
+        const auto spec = [=](int k) -> float2 {
+            const float2 a = buf[k & (half - 1)];
+            const float2 c = buf[(half - k) & (half - 1)];
+            const float2 e = make_float2(0.5f * (a.x + c.x), 0.5f * (a.y - c.y));  // (a + conj c) / 2
+            const float2 o = make_float2(0.5f * (a.y + c.y), 0.5f * (c.x - a.x));  // (a - conj c) / 2i
+            return cmul(cadd(e, cmul(twN.template get<false>(k), o)), H[k]);
+        };
+        const auto fold = [=](int j) -> float2 {
+            float2 acc = spec(j);
+            for (int k = j + M; k <= half; k += M) acc = cadd(acc, spec(k));
+            return acc;
+        };
+        const auto one_sided = [=](int k) -> float2 { return k <= half ? spec(k) : make_float2(0.f, 0.f); };
+        if (p.fold_log2 == 0) fft_fused<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, one_sided, store);
+        else fft_fused<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, fold, store);
+    } else {
+        // complex bins (phase delay) or noise: the full N-point transform
+        const int nrf = p.num_rf;
+        const float amp = p.noise_amplitude;
+        const uint32_t seed = p.noise_seed, counter = p.noise_counter;
+        const auto load = [=](int n) -> float2 {
+            if (n >= nrf) return make_float2(0.f, 0.f);
+            float2 v;
+            if (COMPLEX_BINS) v = static_cast<const float2*>(p.bins)[(size_t)line * nrf + n];
+            else v = make_float2(static_cast<const float*>(p.bins)[(size_t)line * nrf + n], 0.f);
+            if (amp > 0.0f) {
+                const float2 z = normal_pair(seed, counter, (uint32_t)line, (uint32_t)n);
+                v = make_float2(v.x + amp * z.x, v.y + amp * z.y);
+            }
+            return v;
+        };
+        fft_fused<NT, NMAX, false, RMAX, false, true>(buf, N, p.log2_len, twN, load,
+                                                     [buf](int i, float2 v) { buf[i] = v; });
+        const auto fold = [=](int j) -> float2 {
+            float2 acc = make_float2(0.f, 0.f);
+            for (int k = j; k <= half; k += M) acc = cadd(acc, cmul(buf[k], H[k]));
+            return acc;
+        };
+        fft_fused<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, fold, store);
+    }
 }
 
 template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX>
@@ -470,7 +553,8 @@ static hipError_t launch_conv(const ConvParams& p, size_t lds, hipStream_t strea
 hipError_t launch_convolve_demod(const ConvParams& p, hipStream_t stream) {
     if (p.num_lines == 0) return hipSuccess;
     const int nt = conv_block_threads(p.fft_len);
-    const size_t lds = ((size_t)p.fft_len + (p.fft_len >> p.twiddle_bits) + (1u << p.twiddle_bits)) * sizeof(float2);
+    const size_t lds =
+        ((size_t)conv_buffer_len(p) + (p.fft_len >> p.twiddle_bits) + (1u << p.twiddle_bits)) * sizeof(float2);
     const bool cb = p.complex_bins != 0, r8 = p.radix == 8;
     if (nt == 1024) {
         if (r8) return cb ? launch_conv<1024, kMaxFftLen, true, 8>(p, lds, stream)

@@ -752,7 +752,9 @@ struct bcsim_sim {
     void upload_conv_tables() {
         const int N = fft_len;
         // H = mask * FFT_N(excitation) / N in double (BeamConvolver.cpp:77-91, the 1/N of
-        // GpuAlgorithm.cpp:406), by direct summation over the excitation's few samples.
+        // GpuAlgorithm.cpp:406), by direct summation over the excitation's few samples,
+        // times exp(2 pi i k center / N): the spectrum of the output delayed by the
+        // excitation's centre (CpuAlgorithm.cpp:353-358 reads sample center + D j).
         std::vector<float2> H(N);
         const double two_pi = 2.0 * M_PI;
         for (int k = 0; k < N; k++) {
@@ -770,7 +772,9 @@ struct bcsim_sim {
                     im += (double)exc[n] * std::sin(a);
                 }
             }
-            H[k] = make_float2((float)(mask * re / N), (float)(mask * im / N));
+            const double sh = two_pi * (double)(((long long)k * center) % N) / (double)N;
+            const double cs = std::cos(sh), sn = std::sin(sh);
+            H[k] = make_float2((float)(mask * (re * cs - im * sn) / N), (float)(mask * (re * sn + im * cs) / N));
         }
         // demodulation: exp(-2 pi i nfd n), n = j*D, with nfd the reference's float quotient
         // (CpuAlgorithm.cpp:353-358); the cycle count nfd*n is exact in double.
@@ -1328,6 +1332,12 @@ struct bcsim_sim {
         p.center = center;
         p.decimation = decimation;
         p.n_out = n_out;
+        // fold the inverse transform by the largest power of two dividing the
+        // decimation, keeping at least 16 points
+        int f = 0;
+        while (((decimation >> f) & 1) == 0 && log2_len - (f + 1) >= 4) f++;
+        p.fold_log2 = f;
+        p.out_stride = decimation >> f;
         p.iq = d_out;
         p.noise_amplitude = noise_amplitude > 0.0f ? noise_amplitude : 0.0f;
         p.noise_seed = noise_seed;

@@ -446,6 +446,10 @@ void check_conv(const ConvParams& p, const char* op) {
     if (p.n_out > 0 && (long long)p.center + (long long)(p.n_out - 1) * p.decimation >= p.fft_len)
         die(std::string(op) + ": extraction beyond the transform");
     if (p.radix != 4 && p.radix != 8) die(std::string(op) + ": radix");
+    if (p.fold_log2 < 0 || p.log2_len - p.fold_log2 < 4 || (p.out_stride << p.fold_log2) != p.decimation)
+        die(std::string(op) + ": fold");
+    if (p.n_out > 0 && (long long)(p.n_out - 1) * p.out_stride >= (p.fft_len >> p.fold_log2))
+        die(std::string(op) + ": extraction beyond the folded transform");
     if (p.num_rf > p.fft_len) die(std::string(op) + ": num_rf > fft_len");
 }
 }  // namespace

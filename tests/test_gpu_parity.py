@@ -56,6 +56,11 @@ CASES = {
     "spline_degree4": lambda: W.spline_mmode(5000, 8, seed=28, num_cp=12, degree=4),
     "lut_volume_dec2": lambda: W.volume_lut(50000, n_az=6, n_el=6, seed=29),
     "lut_volume_phase": lambda: W.volume_lut(50000, n_az=4, n_el=4, seed=30, phase_delay=True, decimation=1),
+    # decimations whose largest power-of-two factor folds the inverse transform
+    "fixed_gauss_dec1": lambda: W.fixed_sector_2d(20000, 12, seed=31, decimation=1),
+    "fixed_gauss_dec8_short": lambda: W.fixed_sector_2d(8000, 12, seed=32, line_length=0.028, decimation=8),
+    "fixed_gauss_phase_dec6": lambda: W.fixed_sector_2d(20000, 12, seed=33, phase_delay=True, decimation=6),
+    "fixed_gauss_dec16_tiny": lambda: W.fixed_sector_2d(3000, 8, seed=34, line_length=0.002, decimation=16),
 }
 
 
