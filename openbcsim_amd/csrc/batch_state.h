@@ -167,6 +167,7 @@ struct bcsim_batch {
                     J.g = s->tile_launch(t.nchunks, tt);
                 }
                 J.g.block = 1024;
+                J.g.chunks = t.chunk_order;
                 J.nchunks = t.nchunks;
                 J.flags = (s->arc ? kJobArc : 0) | (cplx ? kJobPhase : 0) | (lut ? kJobLut : 0) |
                           (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0);

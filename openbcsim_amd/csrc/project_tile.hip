@@ -202,8 +202,9 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     }
     for (int next = cbeg; next < cend;) {
         // (A) cull the next `step` chunks
-        const int ci = next + (int)threadIdx.x;
-        if ((int)threadIdx.x < step && ci < cend) {
+        const int pos = next + (int)threadIdx.x;
+        if ((int)threadIdx.x < step && pos < cend) {
+            const int ci = g.chunks ? g.chunks[pos] : pos;
             const ChunkBox B = boxes[ci];
             if (!(SHARED && ARC) || chunk_in_window(r_lo, r_hi, tl[0], B)) {
                 unsigned mask = 0;
@@ -341,35 +342,56 @@ __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullPa
     tile_block<ARC, PHASE, LUT, SPLINE, SHARED>(c, cp, fs, ss, boxes, nchunks, out, g, wg, blockIdx.y, lds, list_n);
 }
 
-// Order of the next tile launch's blocks: sorted by the (chunk, line) pairs each
-// listed in the previous launch, heaviest first, so that the heaviest blocks
-// start in the first wave of workgroups and the launch does not end on a long
-// tail; the counts are then cleared for the next launch to fill.  Ties keep
-// the block order.  Bitonic sort of (work << 32 | ~block) keys in LDS.
+// Order of the next tile launch's blocks: by the (chunk, line) pairs each listed
+// in the previous launch, heaviest first, so that the heaviest blocks start in
+// the first wave of workgroups and the launch does not end on a long tail; the
+// counts are then cleared for the next launch to fill.  A counting sort into
+// kOrderBuckets buckets spaced evenly up to the largest count: a few barriers
+// instead of a full sort (order within a bucket is arbitrary).
+constexpr int kOrderBuckets = 256;
+
 __global__ __launch_bounds__(1024) void order_tiles_kernel(int* __restrict__ work, int* __restrict__ order, int n) {
-    __shared__ unsigned long long key[kMaxOrderedTiles];
-    int P = 1;
-    while (P < n) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x)
-        key[i] = i < n ? ((unsigned long long)(unsigned)max(work[i], 0) << 32) | (0xffffffffu - (unsigned)i) : 0ull;
+    __shared__ int count[kOrderBuckets];
+    __shared__ int start[kOrderBuckets];
+    __shared__ int wmax;
+    if (threadIdx.x < kOrderBuckets) count[threadIdx.x] = 0;
+    if (threadIdx.x == 0) wmax = 1;
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const unsigned long long a = key[i], b = key[ixj];
-                    if ((i & k) == 0 ? a < b : a > b) {
-                        key[i] = b;
-                        key[ixj] = a;
-                    }
-                }
-            }
-            __syncthreads();
+    int m = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = max(m, work[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(&wmax, m);
+    __syncthreads();
+    const long long top = wmax;
+    // heaviest first: bucket 0 holds the largest counts
+    const auto bucket = [top](int w) {
+        return (kOrderBuckets - 1) - (int)(((long long)max(w, 0) * (kOrderBuckets - 1)) / top);
+    };
+    for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&count[bucket(work[i])], 1);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // exclusive scan of the counts: four buckets per lane, then across the wave
+        constexpr int Q = kOrderBuckets / 64;
+        const int l = threadIdx.x;
+        int c[Q], sum = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) sum += (c[q] = count[Q * l + q]);
+        int incl = sum;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d);
+            if (l >= d) incl += t;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            start[Q * l + q] = run;
+            run += c[q];
         }
     }
+    __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        order[i] = (int)(0xffffffffu - (unsigned)(key[i] & 0xffffffffull));
+        const int w = work[i];
+        order[atomicAdd(&start[bucket(w)], 1)] = i;
         work[i] = 0;
     }
 }

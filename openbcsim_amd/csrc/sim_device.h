@@ -163,10 +163,11 @@ struct TileLaunch {
     int list_cap;             // chunk list entries in LDS (multiple of 64)
     const int* order;         // workgroup w runs block order[w] (heaviest first), or null: block w
     int* work;                // per-block hit counts added up by the launch, or null
+    const int* chunks;        // chunk visiting order (a permutation of [0, nchunks)), or null: index order
 };
 
-// Blocks one order_tiles launch can sort (64 KiB of LDS keys).
-constexpr int kMaxOrderedTiles = 8192;
+// Blocks one order_tiles launch orders (one workgroup walks them three times).
+constexpr int kMaxOrderedTiles = 65536;
 
 // One projection job of a batch: a (simulator, dataset) pair with everything its
 // tile kernel would receive as arguments.
@@ -211,8 +212,8 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams& cp, const 
                                bool arc, bool phase, bool lut, bool shared_origin, const TileLaunch& cfg,
                                hipStream_t stream);
 size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap);
-// Sorts blocks 0..n-1 (n <= kMaxOrderedTiles) by work[] descending into order[]
-// and clears work[] for the next tile launch to fill.
+// Orders blocks 0..n-1 (n <= kMaxOrderedTiles) by work[], heaviest first, into
+// order[] and clears work[] for the next tile launch to fill.
 hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream);
 
 // Batched frames (one launch each for all jobs of a batch): persistent

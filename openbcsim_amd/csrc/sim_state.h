@@ -5,12 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -70,11 +72,57 @@ struct DevBuf {
 // change the frame beyond float summation order.  A dataset is immutable once
 // built and shared (shared_ptr) by every simulator that projects it: the
 // serving path keeps one resident copy of a phantom for many frame requests.
+// Orders in which the tile kernel visits a dataset's chunks, one per scan
+// origin: by the distance of the chunk's box centre from a common origin, or by
+// its depth along the middle line when the lines do not share one.  Every
+// workgroup then sweeps the dataset from near to far at about the same pace, so
+// the workgroups sharing an L2 read nearby radial slices of the beam-profile
+// table at any one time.  The order never changes the sums.  Orders live as long as the dataset (at
+// most kMaxOrders origins; further origins visit the chunks in index order).
+struct ChunkOrders {
+    using Key = std::array<float, 7>;  // origin, direction (depth mode), shared-origin flag
+    static constexpr size_t kMaxOrders = 16;
+    std::vector<float4> centres;       // box centres; w = 1 marks an empty chunk
+    std::mutex mu;
+    std::vector<std::pair<Key, std::unique_ptr<DevBuf>>> orders;
+
+    void set_boxes(const std::vector<ChunkBox>& boxes) {
+        centres.resize(boxes.size());
+        for (size_t c = 0; c < boxes.size(); c++)
+            centres[c] = make_float4(boxes[c].cx, boxes[c].cy, boxes[c].cz, boxes[c].hx >= 0.0f ? 0.0f : 1.0f);
+    }
+
+    // Device array of chunk indices (a permutation of [0, nchunks)), or null.
+    const int* get(const Key& key) {
+        std::lock_guard<std::mutex> g(mu);
+        for (auto& e : orders)
+            if (e.first == key) return e.second->as<int>();
+        if (orders.size() >= kMaxOrders || centres.empty()) return nullptr;
+        std::vector<std::pair<float, int>> k(centres.size());
+        for (size_t c = 0; c < centres.size(); c++) {
+            const float4 q = centres[c];
+            const float dx = q.x - key[0], dy = q.y - key[1], dz = q.z - key[2];
+            float v = key[6] != 0.0f ? dx * dx + dy * dy + dz * dz : dx * key[3] + dy * key[4] + dz * key[5];
+            if (q.w != 0.0f || !(v == v)) v = INFINITY;  // empty chunks (and NaN keys) last
+            k[c] = {v, (int)c};
+        }
+        std::sort(k.begin(), k.end());
+        std::vector<int> order(k.size());
+        for (size_t c = 0; c < k.size(); c++) order[c] = k[c].second;
+        auto buf = std::make_unique<DevBuf>();
+        buf->ensure(order.size() * sizeof(int));
+        hip_check(hipMemcpy(buf->p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+        orders.emplace_back(key, std::move(buf));
+        return orders.back().second->as<int>();
+    }
+};
+
 struct FixedDataset {
     DevBuf pts;    // float4[n] (x, y, z, amplitude), Morton order
     DevBuf boxes;  // ChunkBox[nchunks]
     size_t n = 0;
     int nchunks = 0;
+    mutable ChunkOrders visit;
 };
 
 struct SplineDataset {
@@ -85,6 +133,7 @@ struct SplineDataset {
     std::vector<float> knots;
     DevBuf cx, cy, cz, amp;  // [num_cp][n] x3, [n], Morton order of the control-point centroid
     DevBuf boxes;            // ChunkBox[nchunks] over all control points of the chunk
+    mutable ChunkOrders visit;
 };
 
 // Per-simulator buffers of one spline dataset that depend on the scan.
@@ -251,6 +300,7 @@ struct bcsim_sim {
     // heaviest first, by the work each block took in this simulator's previous
     // frame (counted by the launch itself); the order never changes the sums.
     bool tile_order = true;
+    bool chunk_order = true;  // visit chunks near to far (ChunkOrders)
     std::vector<std::unique_ptr<DevBuf>> tile_order_bufs;  // per projection target: [work | order]
     int radix = 0;          // FFT pass radix set by the user (4 or 8); 0 = tuned or default
     std::string last_plan;  // launch shapes of the last frame, for get_parameter("launch_plan")
@@ -365,6 +415,8 @@ struct bcsim_sim {
             autotune = parse_bool(value);
         } else if (key == "tile_order") {
             tile_order = parse_bool(value);
+        } else if (key == "chunk_order") {
+            chunk_order = parse_bool(value);
         } else if (key == "fft_radix") {
             const int v = parse_int(value);
             if (v != 0 && v != 4 && v != 8) fail(BCSIM_ERR_INVALID_ARGUMENT, "fft_radix must be 4, 8 or 0 (automatic)");
@@ -409,6 +461,7 @@ struct bcsim_sim {
         if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
         if (key == "autotune") return autotune ? "on" : "off";
         if (key == "tile_order") return tile_order ? "on" : "off";
+        if (key == "chunk_order") return chunk_order ? "on" : "off";
         if (key == "fft_radix") return std::to_string(radix);
         if (key == "launch_plan") return last_plan;
         if (key == "autotune_report") return TuneCache::get().report();
@@ -571,6 +624,7 @@ struct bcsim_sim {
             for (size_t i = (size_t)c * kChunk; i < std::min(n, (size_t)(c + 1) * kChunk); i++) acc.add(&sorted[4 * i]);
             boxes[c] = acc.box();
         }
+        ds->visit.set_boxes(boxes);
         ds->pts.ensure(n * sizeof(float4));
         ds->boxes.ensure(boxes.size() * sizeof(ChunkBox));
         if (n) {
@@ -639,6 +693,7 @@ struct bcsim_sim {
         ds->cy.ensure(b);
         ds->cz.ensure(b);
         ds->amp.ensure(n * sizeof(float));
+        ds->visit.set_boxes(boxes);
         ds->boxes.ensure(boxes.size() * sizeof(ChunkBox));
         if (n) {
             hip_check(hipMemcpy(ds->cx.p, px.data(), b, hipMemcpyHostToDevice), "hipMemcpy");
@@ -1058,10 +1113,20 @@ struct bcsim_sim {
         size_t n = 0;
         const SplineDataset* sp = nullptr;  // spline kinds
         SplineScan* scan = nullptr;
+        const int* chunk_order = nullptr;   // visiting order of the chunks (ChunkOrders), or null
     };
+
+    // Key of this scan's chunk visiting order (see ChunkOrders).
+    ChunkOrders::Key chunk_order_key() const {
+        if (lines.empty()) return ChunkOrders::Key{};
+        if (shared_origin()) return ChunkOrders::Key{lines[0].ox, lines[0].oy, lines[0].oz, 0.f, 0.f, 0.f, 1.f};
+        const LineGeom& m = lines[lines.size() / 2];
+        return ChunkOrders::Key{m.ox, m.oy, m.oz, m.rx, m.ry, m.rz, 0.f};
+    }
 
     std::vector<ProjTarget> proj_targets() {
         std::vector<ProjTarget> out;
+        const ChunkOrders::Key okey = chunk_order_key();
         for (auto& f : fixed) {
             if (f->n == 0) continue;
             ProjTarget t{"fixed", proj_common(false)};
@@ -1069,6 +1134,7 @@ struct bcsim_sim {
             t.boxes = f->boxes.as<ChunkBox>();
             t.nchunks = f->nchunks;
             t.n = f->n;
+            if (chunk_order) t.chunk_order = f->visit.get(okey);
             out.push_back(t);
         }
         for (size_t q = 0; q < splines.size(); q++) {
@@ -1087,6 +1153,7 @@ struct bcsim_sim {
             t.n = sp->n;
             t.sp = sp;
             t.scan = scan;
+            if (chunk_order) t.chunk_order = sp->visit.get(okey);
             out.push_back(t);
         }
         return out;
@@ -1133,8 +1200,10 @@ struct bcsim_sim {
         return b->as<int>();
     }
 
-    void launch_target(const ProjTarget& t, const TileLaunch& cfg, float* bins) {
+    void launch_target(const ProjTarget& t, const TileLaunch& cfg_in, float* bins) {
         const bool is_spline = std::strcmp(t.kind, "spline") == 0;
+        TileLaunch cfg = cfg_in;
+        cfg.chunks = t.chunk_order;
         hip_check(launch_project_tile(t.c, cull_params(t.c), is_spline ? nullptr : &t.fs, is_spline ? &t.ss : nullptr,
                                       t.boxes, t.nchunks, bins, arc, phase_delay, profile == kProfileLut,
                                       shared_origin() && !is_spline, cfg, stream),

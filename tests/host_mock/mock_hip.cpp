@@ -420,6 +420,15 @@ void check_tile(const ProjCommon& c, const FixedSet* fs, const SplineSet* ss, co
     if (block < 64 || block > 1024 || block % 64) die(std::string(op) + ": block");
     const size_t lds = tile_lds_bytes(g.window_bins, phase, g.tile_lines, g.list_cap);
     if (lds + 8 > kLds) die(std::string(op) + ": LDS over 160 KiB");
+    if (g.chunks) {
+        check(g.chunks, (size_t)nchunks * 4, "chunk order", op);
+        std::vector<char> seen(nchunks, 0);
+        for (int i = 0; i < nchunks; i++) {
+            const int ci = g.chunks[i];
+            if (ci < 0 || ci >= nchunks || seen[ci]) die(std::string(op) + ": chunk order is not a permutation");
+            seen[ci] = 1;
+        }
+    }
     const int blocks = ((c.num_lines + g.tile_lines - 1) / g.tile_lines) * g.windows;
     if (g.work) check(g.work, (size_t)blocks * 4, "work", op);
     if (g.order) {

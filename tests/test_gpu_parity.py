@@ -295,9 +295,10 @@ def test_scan_length_changes_between_frames():
 
 def test_tile_order_keeps_frames():
     """Blocks of large tile launches run heaviest first, ordered by the previous
-    frame's counts: the frames must not change beyond float summation order, from
-    the first frame (no counts yet) on, when the number of blocks changes between
-    frames, and with several datasets (one order per dataset)."""
+    frame's counts, and visit the chunks near to far from the scan origin: the
+    frames must not change beyond float summation order, from the first frame
+    (no counts yet) on, when the number of blocks changes between frames, and
+    with several datasets (one order per dataset)."""
     from openbcsim_amd.pyrfsim import RfSimulator
     wl = W.volume_lut(300_000, n_az=64, n_el=40, seed=41)
     wl.fixed.append(W.volume_lut(200_000, n_az=2, n_el=2, seed=42).fixed[0])
@@ -305,8 +306,9 @@ def test_tile_order_keeps_frames():
     for mode in ("off", "on"):
         s = RfSimulator("gpu")
         wl.apply(s)
-        s.set_parameter("tile_order", mode)
-        assert s.get_parameter("tile_order") == mode
+        for key in ("tile_order", "chunk_order"):
+            s.set_parameter(key, mode)
+            assert s.get_parameter(key) == mode
         sims[mode] = s
     o_, d, l, t, length = wl.scan
     for frame, keep in enumerate([2560, 2560, 2560, 2048, 2560, 640]):
