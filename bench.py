@@ -44,7 +44,7 @@ BYTES_PER_EVAL = {"fixed": 16, "spline": 52}
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, the scales a 256 MiB device copy reads back as), for the committed kernels;
 # profiles/project_tile_volume_lut_1M_4096_pmc.md gives the commands and counts.
-MEASURED_TRAFFIC = {"volume_lut": {"bytes": 3.704e6 * 1024 * 2.0 + 1.574e5 * 1024,
+MEASURED_TRAFFIC = {"volume_lut": {"bytes": 1.867e6 * 1024 * 2.0 + 1.573e5 * 1024,
                                    "source": "profiles/project_tile_volume_lut_1M_4096_pmc.md"}}
 
 
@@ -408,7 +408,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="volume_lut")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched-vs-per-call leg")

@@ -65,7 +65,11 @@ BCSIM_API bcsim_status bcsim_destroy(bcsim_sim* sim);
  * pair, for checking); autotune (on/off, default off: time candidate launch
  * shapes once per shape class, process-wide, and use the fastest); fft_radix
  * (4 or 8; 0 = tuned or default); tile_lines, tile_block, tile_slices,
- * tile_windows (tile-kernel launch shape; 0 = tuned or heuristic).
+ * tile_windows (tile-kernel launch shape; 0 = tuned or heuristic); tile_order
+ * (on/off, default on: large projection launches run their blocks heaviest
+ * first, by the previous frame's counts); chunk_order (on/off, default on: the
+ * projection visits the chunks near to far from the scan origin).  Neither
+ * changes a frame beyond float summation order.
  * Read-only keys of bcsim_get_parameter: launch_plan (shapes of the last frame)
  * and autotune_report (every tuned class with its trial timings). */
 BCSIM_API bcsim_status bcsim_set_parameter(bcsim_sim* sim, const char* key, const char* value);
