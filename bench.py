@@ -213,11 +213,11 @@ def run_serving(seconds: float = 10.0, rate: float = 200.0, scale: float = 1.0, 
     """Serving loop: 16 resident phantoms (spline M-mode, fixed 2D, 1M-point LUT
     volumes, uneven batched-mix jobs), geometries of 32-512 sector lines and the
     4096-line pyramid; Poisson arrivals at ``rate`` requests/s for ``seconds``,
-    each naming a phantom, a geometry and a time t.  Every (phantom, geometry)
-    pair is requested once before the timed run (shape tuning and buffer set-up
-    happen there).  About 1% of the timed requests are re-simulated alone on a
-    single simulator and compared (rel L2)."""
-    import threading
+    each naming a phantom, a geometry and a time t, run twice: into pinned
+    output buffers of the server's (the reported p50/p99) and into fresh numpy
+    arrays.  Every (phantom, geometry) pair is requested once before the timed
+    runs (shape tuning and buffer set-up happen there).  About 1% of the timed
+    requests are re-simulated alone on a single simulator and compared (rel L2)."""
     from openbcsim_amd import workloads as W
     from openbcsim_amd.pyrfsim import RfSimulator
     from openbcsim_amd.serve import Server
@@ -239,21 +239,70 @@ def run_serving(seconds: float = 10.0, rate: float = 200.0, scale: float = 1.0, 
     for h in warm:
         srv.wait(h)
     warm_s = time.perf_counter() - t0
-    srv.set_parameter("reset_stats", 1)
+    pinned = poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_outputs=True)
+    pageable = poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_outputs=False)
+    sat = saturated_serving(srv, sims, geometries, pairs, rng)
+    srv.close()
+    for s in sims:
+        s.close()
+    out = dict(pinned)
+    out.update({"outputs": "pinned (Server.alloc_output: IQ written by DMA, no copy in wait)",
+                "pageable_outputs": {k: pageable[k] for k in ("requests", "p50_latency_ms", "p99_latency_ms",
+                                                              "mean_latency_ms", "latency_by_lines",
+                                                              "copy_to_caller_ms", "sampled_rel_err_vs_single")},
+                "phantoms": len(phantoms), "geometries": len(geometries), "pairs": len(pairs),
+                "warmup_s": round(warm_s, 2), "saturated": sat})
+    return out
 
+
+def poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_outputs):
+    """One timed run of Poisson arrivals; latency = submit to the IQ in the
+    caller's buffer.  pinned_outputs: the caller's buffers come from
+    Server.alloc_output (recycled after each wait), otherwise fresh numpy arrays."""
+    import threading
+    from openbcsim_amd import workloads as W
+    from openbcsim_amd.pyrfsim import RfSimulator
+    srv.set_parameter("reset_stats", 1)
     n = max(1, int(rng.poisson(rate * seconds)))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     choice = rng.integers(len(pairs), size=n)
     times = rng.uniform(0.0, 1.0, n).astype(np.float32)
+    sample = set(int(i) for i in rng.choice(n, max(1, n // 100), replace=False))
+    kept = {}
+    pool, pool_lock, allocated = {}, threading.Lock(), []
+
+    def take(p, g):
+        if not pinned_outputs:
+            return None
+        with pool_lock:
+            free = pool.setdefault((p, g), [])
+            if free:
+                return free.pop()
+        arr = srv.alloc_output(p, g)
+        arr.fill(0)  # touch the pages before the timed use
+        allocated.append(arr)
+        return arr
+
+    if pinned_outputs:  # two buffers per pair beforehand, more on demand
+        for p, g in pairs:
+            bufs = [take(p, g), take(p, g)]
+            pool.setdefault((p, g), []).extend(bufs)
     handles = [None] * n
+    lat_by = {}
     ready = threading.Semaphore(0)
-    done = []
 
     def waiter():
         for i in range(n):
             ready.acquire()
-            srv.wait(handles[i])
-            done.append(i)
+            h = handles[i]
+            srv.wait(h)
+            p, g = pairs[int(choice[i])]
+            lat_by.setdefault(int(h.out.shape[0]), []).append((h.latency_ms, h.out.nbytes))
+            if i in sample:
+                kept[i] = h.out.copy()
+            if pinned_outputs:
+                with pool_lock:
+                    pool[(p, g)].append(h.out)
 
     th = threading.Thread(target=waiter)
     th.start()
@@ -266,39 +315,41 @@ def run_serving(seconds: float = 10.0, rate: float = 200.0, scale: float = 1.0, 
         else:
             late = max(late, now - arrivals[i])
         p, g = pairs[int(choice[i])]
-        handles[i] = srv.submit(p, g, float(times[i]))
+        handles[i] = srv.submit(p, g, float(times[i]), out=take(p, g))
         ready.release()
     th.join()
     wall = time.perf_counter() - start
     lat = srv.stats("latency_ms")
+    copy_ms = srv.stats("copy_ms")
     jobs = srv.stats("frame_jobs")
     lines = float(srv.stats("lines_completed")[0])
+    latency_by_lines = {str(k): {"requests": len(v),
+                                 "p50_ms": round(float(np.percentile([x[0] for x in v], 50)), 3),
+                                 "p99_ms": round(float(np.percentile([x[0] for x in v], 99)), 3),
+                                 "iq_mb": round(float(np.mean([x[1] for x in v])) / 1e6, 2)}
+                        for k, v in sorted(lat_by.items())}
     # ~1% of the requests again, alone on a single simulator
-    sample = rng.choice(n, max(1, n // 100), replace=False)
     worst = 0.0
-    for i in sample:
+    for i in sorted(sample):
         p, g = pairs[int(choice[i])]
         s = RfSimulator("gpu")
         W.at_time(phantoms[p], geometries[g], float(times[i])).apply(s)
         ref = s.simulate_frame()
         s.close()
-        out = handles[i].out
-        worst = max(worst, float(np.linalg.norm(out - ref) / max(np.linalg.norm(ref), 1e-30)))
-    sat = saturated_serving(srv, sims, geometries, pairs, rng)
-    srv.close()
-    for s in sims:
-        s.close()
+        worst = max(worst, float(np.linalg.norm(kept[i] - ref) / max(np.linalg.norm(ref), 1e-30)))
+    for arr in allocated:
+        srv.free_output(arr)
     return {"requests": n, "seconds": round(wall, 3), "offered_rate_per_s": rate,
             "achieved_rate_per_s": round(n / wall, 1), "max_submit_lag_ms": round(late * 1e3, 2),
             "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_latency_ms": round(float(np.mean(lat)), 3),
+            "latency_by_lines": latency_by_lines,
+            "copy_to_caller_ms": {"p50": round(float(np.percentile(copy_ms, 50)), 3),
+                                  "p99": round(float(np.percentile(copy_ms, 99)), 3)},
             "iq_lines_per_s": round(lines / wall, 1), "frames": int(len(jobs)),
             "mean_requests_per_frame": round(float(np.mean(jobs)), 2),
-            "phantoms": len(phantoms), "geometries": len(geometries), "pairs": len(pairs),
-            "warmup_s": round(warm_s, 2),
-            "sampled_rel_err_vs_single": {"requests": int(len(sample)), "max": worst},
-            "saturated": sat}
+            "sampled_rel_err_vs_single": {"requests": int(len(sample)), "max": worst}}
 
 
 def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300, clients: int = 8):

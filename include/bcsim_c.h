@@ -239,7 +239,9 @@ BCSIM_API bcsim_status bcsim_server_add_geometry(bcsim_server* server, int num_l
 BCSIM_API bcsim_status bcsim_server_get_output_shape(bcsim_server* server, int phantom_id, int geometry_id,
                                                      int* num_lines, int* num_samples);
 /* Queue one frame; out (complex64 [lines][samples], capacity complex elements)
- * must stay valid until bcsim_server_wait reports the request. */
+ * must stay valid until bcsim_server_wait reports the request.  When out lies in
+ * a buffer of bcsim_server_alloc_host, the IQ is copied there by DMA and wait
+ * copies nothing. */
 BCSIM_API bcsim_status bcsim_server_submit(bcsim_server* server, int phantom_id, int geometry_id, float t,
                                            float* out, size_t capacity, uint64_t* request_id);
 /* Wait (timeout_ms < 0: without limit) for a request; returns its status and
@@ -256,6 +258,12 @@ BCSIM_API bcsim_status bcsim_server_wait(bcsim_server* server, uint64_t request_
  * idle_slots, pending (one value each).  out NULL: count only. */
 BCSIM_API bcsim_status bcsim_server_get_stats(bcsim_server* server, const char* key, double* out,
                                               size_t capacity, size_t* count);
+/* Pinned host memory for request outputs (no counterpart in the reference): the
+ * frame's IQ goes straight into it by DMA, without the staging copy.  Freed by
+ * bcsim_server_free_host (an error while an unreported request writes into it)
+ * or with the server. */
+BCSIM_API bcsim_status bcsim_server_alloc_host(bcsim_server* server, size_t bytes, void** ptr);
+BCSIM_API bcsim_status bcsim_server_free_host(bcsim_server* server, void* ptr);
 
 #ifdef __cplusplus
 }

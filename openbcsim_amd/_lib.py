@@ -63,6 +63,8 @@ SIGNATURES = {
     "bcsim_server_submit": [_h, C.c_int, C.c_int, C.c_float, _fp, C.c_size_t, C.POINTER(C.c_uint64)],
     "bcsim_server_wait": [_h, C.c_uint64, C.c_int, _ip, _ip, _dp],
     "bcsim_server_get_stats": [_h, C.c_char_p, _dp, C.c_size_t, _sp],
+    "bcsim_server_alloc_host": [_h, C.c_size_t, C.POINTER(C.c_void_p)],
+    "bcsim_server_free_host": [_h, C.c_void_p],
 }
 
 _lib = None

@@ -131,6 +131,7 @@ struct Request {
     Slot* slot = nullptr;
     void* staged = nullptr;  // pinned copy of the IQ
     size_t staged_cls = 0;
+    bool direct = false;     // `out` is pinned memory of the server's: the IQ is copied there by DMA
 };
 
 struct Shape {
@@ -138,17 +139,20 @@ struct Shape {
     size_t slot_bytes = 0;
 };
 
-// One frame context: a batch, its device output, and the event that marks the
-// frame's IQ staged in pinned host memory.
+// One frame context: a batch, its device output, and per request (in the order
+// their IQ is copied to host memory, smallest first) the event that marks that
+// request's IQ in host memory, so a small request does not wait for the copies
+// of larger ones.
 struct Frame {
     bcsim_batch batch;
     DevBuf d_out;
     size_t out_accounted = 0;  // bytes of d_out counted in the server's `used`
-    hipEvent_t done = nullptr;
-    std::vector<Request*> reqs;
+    std::vector<hipEvent_t> events;  // reused; events[k] follows the copy of copy_order[k]
+    std::vector<Request*> reqs;      // in batch (job) order
+    std::vector<Request*> copy_order;
     bool busy = false;
     ~Frame() {
-        if (done) (void)hipEventDestroy(done);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
 };
 
@@ -172,6 +176,7 @@ struct bcsim_server {
     uint64_t completions = 0;
 
     PinnedPool pinned;
+    std::map<const char*, size_t> host_blocks;  // pinned caller buffers (bcsim_server_alloc_host)
     std::vector<std::unique_ptr<bcsim_sim>> phantoms;  // snapshots (views) of the callers' simulators
     std::set<const void*> resident_sets;               // datasets and LUTs already accounted
     std::vector<Geometry> geometries;
@@ -199,6 +204,7 @@ struct bcsim_server {
         if (scheduler.joinable()) scheduler.join();
         if (completer.joinable()) completer.join();
         for (auto& kv : requests) pinned.give(kv.second->staged, kv.second->staged_cls);
+        for (auto& kv : host_blocks) (void)hipHostFree(const_cast<char*>(kv.first));
     }
 
     void start() {
@@ -301,6 +307,7 @@ struct bcsim_server {
         r->capacity = capacity;
         r->lines = sh.lines;
         r->samples = sh.samples;
+        r->direct = in_host_block(out, (size_t)sh.lines * sh.samples * sizeof(float2));
         r->t_submit = now_ms();
         if (t_first_submit < 0) t_first_submit = r->t_submit;
         pending.push_back(r.get());
@@ -330,10 +337,11 @@ struct bcsim_server {
             pinned.give(r->staged, r->staged_cls);
             fail(status, err);
         }
-        // copy out in this thread, without the lock
+        // copy out in this thread, without the lock (nothing to copy when the DMA
+        // wrote the caller's pinned buffer)
         lk.unlock();
         const double c0 = now_ms();
-        parallel_copy(r->out, r->staged, (size_t)r->lines * r->samples * sizeof(float2));
+        if (!r->direct) parallel_copy(r->out, r->staged, (size_t)r->lines * r->samples * sizeof(float2));
         const double copy = now_ms() - c0;
         pinned.give(r->staged, r->staged_cls);
         const double lat = r->t_done - r->t_submit + copy;  // as seen by a caller waiting from submit
@@ -371,6 +379,42 @@ struct bcsim_server {
                 }
         }
         return true;
+    }
+
+    // Is [p, p + bytes) inside one pinned block of bcsim_server_alloc_host? (mu held)
+    bool in_host_block(const void* p, size_t bytes) const {
+        const char* c = static_cast<const char*>(p);
+        auto it = host_blocks.upper_bound(c);
+        if (it == host_blocks.begin()) return false;
+        --it;
+        return c >= it->first && c + bytes <= it->first + it->second;
+    }
+
+    void* alloc_host(size_t bytes) {
+        if (bytes == 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "zero-byte host buffer");
+        hip_check(hipSetDevice(device < 0 ? 0 : device), "hipSetDevice");
+        void* p = nullptr;
+        hip_check(hipHostMalloc(&p, bytes, 0), "hipHostMalloc");
+        std::lock_guard<std::mutex> g(mu);
+        host_blocks[static_cast<const char*>(p)] = bytes;
+        return p;
+    }
+
+    void free_host(void* p) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            auto it = host_blocks.find(static_cast<const char*>(p));
+            if (it == host_blocks.end()) fail(BCSIM_ERR_INVALID_ARGUMENT, "not a host buffer of this server");
+            const char* lo = it->first;
+            const char* hi = lo + it->second;
+            for (auto& kv : requests) {
+                const char* o = reinterpret_cast<const char*>(kv.second->out);
+                if (o >= lo && o < hi)
+                    fail(BCSIM_ERR_INVALID_STATE, "host buffer still holds the output of an unreported request");
+            }
+            host_blocks.erase(it);
+        }
+        (void)hipHostFree(p);
     }
 
     void finish(Request* r, int status, const std::string& err, double t) {
@@ -517,17 +561,30 @@ struct bcsim_server {
             f->d_out.ensure(total * sizeof(float2));
         }
         for (Request* r : f->reqs)
-            if (!r->staged) r->staged = pinned.take((size_t)r->lines * r->samples * sizeof(float2), r->staged_cls);
-        if (!f->done) hip_check(hipEventCreateWithFlags(&f->done, hipEventDisableTiming), "hipEventCreate");
-        b.simulate_device(f->d_out.p, total);
-        for (size_t i = 0; i < f->reqs.size(); i++) {
-            Request* r = f->reqs[i];
-            const float2* src = f->d_out.as<float2>() + b.plan[i].out_offset;
-            hip_check(hipMemcpyAsync(r->staged, src, (size_t)r->lines * r->samples * sizeof(float2),
-                                     hipMemcpyDeviceToHost, b.stream),
-                      "hipMemcpyAsync");
+            if (!r->staged && !r->direct)
+                r->staged = pinned.take((size_t)r->lines * r->samples * sizeof(float2), r->staged_cls);
+        while (f->events.size() < f->reqs.size()) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            f->events.push_back(e);
         }
-        hip_check(hipEventRecord(f->done, b.stream), "hipEventRecord");
+        b.simulate_device(f->d_out.p, total);
+        // IQ to host memory, smallest first, an event after each request's copy
+        std::vector<size_t> idx(f->reqs.size());
+        for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) {
+            return (size_t)f->reqs[a]->lines * f->reqs[a]->samples < (size_t)f->reqs[c]->lines * f->reqs[c]->samples;
+        });
+        f->copy_order.clear();
+        for (size_t k = 0; k < idx.size(); k++) {
+            Request* r = f->reqs[idx[k]];
+            const float2* src = f->d_out.as<float2>() + b.plan[idx[k]].out_offset;
+            hip_check(hipMemcpyAsync(r->direct ? (void*)r->out : r->staged, src,
+                                     (size_t)r->lines * r->samples * sizeof(float2), hipMemcpyDeviceToHost, b.stream),
+                      "hipMemcpyAsync");
+            hip_check(hipEventRecord(f->events[k], b.stream), "hipEventRecord");
+            f->copy_order.push_back(r);
+        }
         std::lock_guard<std::mutex> g(mu);
         used = used - f->out_accounted + f->d_out.bytes;
         f->out_accounted = f->d_out.bytes;
@@ -544,11 +601,13 @@ struct bcsim_server {
             cv_comp.wait(lk, [&] { return !inflight.empty() || sched_exited; });
             if (inflight.empty()) break;
             Frame* f = inflight.front();
-            lk.unlock();
-            const hipError_t e = hipEventSynchronize(f->done);
-            const double t = now_ms();
-            lk.lock();
-            for (Request* r : f->reqs) {
+            // requests complete one by one, as their copies land
+            for (size_t k = 0; k < f->copy_order.size(); k++) {
+                Request* r = f->copy_order[k];
+                lk.unlock();
+                const hipError_t e = hipEventSynchronize(f->events[k]);
+                const double t = now_ms();
+                lk.lock();
                 if (e == hipSuccess) {
                     finish(r, BCSIM_OK, "", t);
                     queue_ms.push_back(r->t_launch - r->t_submit);
@@ -557,8 +616,10 @@ struct bcsim_server {
                 } else {
                     finish(r, BCSIM_ERR_DEVICE, std::string("frame failed: ") + hipGetErrorString(e), t);
                 }
+                cv_done.notify_all();
             }
             f->reqs.clear();
+            f->copy_order.clear();
             f->busy = false;
             inflight.pop_front();
             completions++;
@@ -690,6 +751,20 @@ BCSIM_API bcsim_status bcsim_server_get_stats(bcsim_server* s, const char* key, 
         if (count) *count = v.size();
         const size_t n = std::min(capacity, v.size());
         if (out && n) std::memcpy(out, v.data(), n * sizeof(double));
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_alloc_host(bcsim_server* s, size_t bytes, void** ptr) {
+    return guarded([&] {
+        if (!s || !ptr) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        *ptr = s->alloc_host(bytes);
+    });
+}
+
+BCSIM_API bcsim_status bcsim_server_free_host(bcsim_server* s, void* ptr) {
+    return guarded([&] {
+        if (!s || !ptr) fail(BCSIM_ERR_INVALID_ARGUMENT, "null argument");
+        s->free_host(ptr);
     });
 }
 

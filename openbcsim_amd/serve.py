@@ -78,6 +78,22 @@ class Server:
         check(self._lib.bcsim_server_get_output_shape(self._h, phantom, geometry, C.byref(nl), C.byref(ns)))
         return nl.value, ns.value
 
+    def alloc_output(self, phantom: int = None, geometry: int = None, shape=None) -> np.ndarray:
+        """Pinned complex64 array (lines, samples) for submit(out=...): the frame's IQ
+        is written into it by DMA, with no staging copy in wait.  Valid until
+        free_output(array) or close()."""
+        nl, ns = shape if shape is not None else self.output_shape(phantom, geometry)
+        n = int(nl) * int(ns)
+        p = C.c_void_p()
+        check(self._lib.bcsim_server_alloc_host(self._h, n * 8, C.byref(p)))
+        flat = np.ctypeslib.as_array((C.c_float * (2 * n)).from_address(p.value))
+        return flat.view(np.complex64).reshape(int(nl), int(ns))
+
+    def free_output(self, arr: np.ndarray):
+        """Release an array of alloc_output (an error while an unreported request
+        writes into it)."""
+        check(self._lib.bcsim_server_free_host(self._h, C.c_void_p(arr.__array_interface__["data"][0])))
+
     def submit(self, phantom: int, geometry: int, t: float = 0.0, out: np.ndarray = None) -> Pending:
         nl, ns = self.output_shape(phantom, geometry)
         if out is None:
