@@ -360,14 +360,14 @@ def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300, clients: 
     synchronous simulate_lines into host memory, the reference's call pattern
     (MainWindow.cpp:666-695) -- with the reference's projection method (one
     thread per scatterer-line pair, "direct") and with the culled tile kernel.
-    Both forms write into output buffers allocated and touched beforehand."""
+    Both forms write into pinned output buffers (Server.alloc_output) allocated
+    and touched beforehand."""
     import threading
     reqs = [(*pairs[int(rng.integers(len(pairs)))], float(rng.uniform(0, 1))) for _ in range(n)]
     lines = sum(geometries[g][0].shape[0] for _, g, _ in reqs)
     outs = []
     for p, g, _ in reqs:
-        nl, ns = srv.output_shape(p, g)
-        buf = np.empty((nl, ns), np.complex64)
+        buf = srv.alloc_output(p, g)
         buf.fill(0)
         outs.append(buf)
 
@@ -401,7 +401,8 @@ def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300, clients: 
             sims[p].set_scan_sequence(o, d, length, l, np.full(o.shape[0], t, np.float32), True)
             key = (p, g)
             if key not in shapes:
-                shapes[key] = np.zeros(sims[p].output_shape(), np.complex64)
+                shapes[key] = srv.alloc_output(shape=sims[p].output_shape())
+                shapes[key].fill(0)
             sims[p].simulate_frame(out=shapes[key])
         return time.perf_counter() - t0
 
@@ -410,6 +411,8 @@ def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300, clients: 
     direct = loop("direct")
     for s in sims:
         s.set_parameter("projection_mode", "tiled")
+    for buf in outs + list(shapes.values()):
+        srv.free_output(buf)
     return {"requests": n, "lines": lines, "clients": clients, "served_lines_per_s": round(lines / served, 1),
             "first_burst_lines_per_s": round(lines / cold, 1),
             "frames": int(len(jobs)), "mean_requests_per_frame": round(float(np.mean(jobs)), 2),
