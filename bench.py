@@ -360,66 +360,76 @@ def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300, clients: 
     synchronous simulate_lines into host memory, the reference's call pattern
     (MainWindow.cpp:666-695) -- with the reference's projection method (one
     thread per scatterer-line pair, "direct") and with the culled tile kernel.
-    Both forms write into pinned output buffers (Server.alloc_output) allocated
-    and touched beforehand."""
+    The served time is the median of three bursts after a first one.
+    Measured twice, each time with the same kind of output buffers for the
+    server and the per-call loops, allocated and touched beforehand: pinned
+    buffers of the server's (Server.alloc_output; the reported figures) and
+    numpy arrays."""
     import threading
     reqs = [(*pairs[int(rng.integers(len(pairs)))], float(rng.uniform(0, 1))) for _ in range(n)]
     lines = sum(geometries[g][0].shape[0] for _, g, _ in reqs)
-    outs = []
-    for p, g, _ in reqs:
-        buf = srv.alloc_output(p, g)
-        buf.fill(0)
-        outs.append(buf)
 
-    def burst():
-        srv.set_parameter("reset_stats", 1)
-        t0 = time.perf_counter()
-        handles = [srv.submit(p, g, t, out=o) for (p, g, t), o in zip(reqs, outs)]
+    def measure(pinned):
+        def buffer(shape):
+            buf = srv.alloc_output(shape=shape) if pinned else np.empty(shape, np.complex64)
+            buf.fill(0)
+            return buf
 
-        def client(k):
-            for h in handles[k::clients]:
-                srv.wait(h)
+        outs = [buffer(srv.output_shape(p, g)) for p, g, _ in reqs]
 
-        threads = [threading.Thread(target=client, args=(k,)) for k in range(clients)]
-        for th in threads:
-            th.start()
-        for th in threads:
-            th.join()
-        return time.perf_counter() - t0
+        def burst():
+            srv.set_parameter("reset_stats", 1)
+            t0 = time.perf_counter()
+            handles = [srv.submit(p, g, t, out=o) for (p, g, t), o in zip(reqs, outs)]
 
-    cold = burst()    # the first burst of this depth also sizes the server's pinned staging pool
-    served = burst()
-    jobs = srv.stats("frame_jobs")
-    shapes = {}
+            def client(k):
+                for h in handles[k::clients]:
+                    srv.wait(h)
 
-    def loop(mode):
+            threads = [threading.Thread(target=client, args=(k,)) for k in range(clients)]
+            for th in threads:
+                th.start()
+            for th in threads:
+                th.join()
+            return time.perf_counter() - t0
+
+        cold = burst()    # the first burst of this depth also sizes the server's pinned staging pool
+        served = float(np.median([burst() for _ in range(3)]))
+        jobs = srv.stats("frame_jobs")
+        shapes = {}
+
+        def loop(mode):
+            for s in sims:
+                s.set_parameter("projection_mode", mode)
+            t0 = time.perf_counter()
+            for p, g, t in reqs:
+                o, d, l, length = geometries[g]
+                sims[p].set_scan_sequence(o, d, length, l, np.full(o.shape[0], t, np.float32), True)
+                key = (p, g)
+                if key not in shapes:
+                    shapes[key] = buffer(sims[p].output_shape())
+                sims[p].simulate_frame(out=shapes[key])
+            return time.perf_counter() - t0
+
+        loop("tiled")  # set-up of every (phantom, geometry) shape and output buffer
+        tiled = loop("tiled")
+        direct = loop("direct")
         for s in sims:
-            s.set_parameter("projection_mode", mode)
-        t0 = time.perf_counter()
-        for p, g, t in reqs:
-            o, d, l, length = geometries[g]
-            sims[p].set_scan_sequence(o, d, length, l, np.full(o.shape[0], t, np.float32), True)
-            key = (p, g)
-            if key not in shapes:
-                shapes[key] = srv.alloc_output(shape=sims[p].output_shape())
-                shapes[key].fill(0)
-            sims[p].simulate_frame(out=shapes[key])
-        return time.perf_counter() - t0
+            s.set_parameter("projection_mode", "tiled")
+        if pinned:
+            for buf in outs + list(shapes.values()):
+                srv.free_output(buf)
+        return {"served_lines_per_s": round(lines / served, 1), "first_burst_lines_per_s": round(lines / cold, 1),
+                "frames": int(len(jobs)), "mean_requests_per_frame": round(float(np.mean(jobs)), 2),
+                "per_call_lines_per_s": round(lines / tiled, 1),
+                "per_call_direct_lines_per_s": round(lines / direct, 1),
+                "speedup_vs_plain_per_call": round(direct / served, 3),
+                "speedup_vs_tiled_per_call": round(tiled / served, 3)}
 
-    loop("tiled")  # set-up of every (phantom, geometry) shape and output buffer
-    tiled = loop("tiled")
-    direct = loop("direct")
-    for s in sims:
-        s.set_parameter("projection_mode", "tiled")
-    for buf in outs + list(shapes.values()):
-        srv.free_output(buf)
-    return {"requests": n, "lines": lines, "clients": clients, "served_lines_per_s": round(lines / served, 1),
-            "first_burst_lines_per_s": round(lines / cold, 1),
-            "frames": int(len(jobs)), "mean_requests_per_frame": round(float(np.mean(jobs)), 2),
-            "per_call_lines_per_s": round(lines / tiled, 1),
-            "per_call_direct_lines_per_s": round(lines / direct, 1),
-            "speedup_vs_plain_per_call": round(direct / served, 3),
-            "speedup_vs_tiled_per_call": round(tiled / served, 3)}
+    res = measure(pinned=True)
+    res.update({"requests": n, "lines": lines, "clients": clients, "outputs": "pinned",
+                "numpy_outputs": measure(pinned=False)})
+    return res
 
 
 def cpu_baseline(wl, target_s: float = 12.0):

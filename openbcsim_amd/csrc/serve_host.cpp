@@ -12,14 +12,17 @@
 //     for the next request of the same phantom and geometry, or evicted least
 //     recently used first), and launches the admitted requests as one batched
 //     frame (one persistent projection launch, one persistent convolution
-//     launch) on one of two frame streams, so the next batch is formed and
-//     launched while the previous one runs;
-//   - each request's IQ is copied by DMA into its own pinned staging buffer at
-//     the end of its frame; a completion thread waits for frames in launch
-//     order and releases their frame buffers, which admits waiting requests;
-//   - the caller's wait copies the staged IQ into the caller's buffer, in the
-//     caller's thread (several threads for large frames), so one large result
-//     does not hold up the completion of the frames behind it.
+//     launch) on one of kFrameContexts frame streams, so the next batches are
+//     formed and launched while earlier ones compute or copy;
+//   - at the end of its frame each request's IQ is copied by DMA, smallest
+//     first, straight into the caller's buffer when that is pinned memory of the
+//     server's (bcsim_server_alloc_host), otherwise into a pinned staging
+//     buffer; a completion thread completes the requests one by one as their
+//     copies land, and releases each frame's buffers, which admits waiting
+//     requests;
+//   - for a staged request the caller's wait copies the IQ into the caller's
+//     buffer, in the caller's thread (several threads for large frames), so one
+//     large result does not hold up the completion of the frames behind it.
 // Each request has its own bins and tables, so its result does not depend on
 // what it is batched with, up to float summation order.
 #include "batch_state.h"
@@ -184,7 +187,8 @@ struct bcsim_server {
     std::vector<std::unique_ptr<Slot>> slots;
     std::deque<Request*> pending;
     std::map<uint64_t, std::unique_ptr<Request>> requests;
-    Frame frames[2];
+    static constexpr int kFrameContexts = 2;
+    Frame frames[kFrameContexts];
     std::deque<Frame*> inflight;
 
     // statistics (completion order)
