@@ -39,6 +39,21 @@
 
 namespace bcsim_hip {
 
+// Can a point of box B lie inside line L's lateral and elevational support
+// (LUT extent; for the Gaussian, where expf is not yet zero)?  dx, dy, dz: box
+// centre minus line origin; m: the margin.
+template <bool LUT>
+__device__ __forceinline__ bool lateral_hits(const CullParams& cp, const LineGeom& L, const ChunkBox& B, float dx,
+                                             float dy, float dz, float m) {
+    const float lc = dx * L.lx + dy * L.ly + dz * L.lz;
+    const float hl = B.hx * fabsf(L.lx) + B.hy * fabsf(L.ly) + B.hz * fabsf(L.lz) + m;
+    const float ec = dx * L.ex + dy * L.ey + dz * L.ez;
+    const float he = B.hx * fabsf(L.ex) + B.hy * fabsf(L.ey) + B.hz * fabsf(L.ez) + m;
+    if (LUT) return (lc + hl > cp.l_lo) && (lc - hl < cp.l_hi) && (ec + he > cp.e_lo) && (ec - he < cp.e_hi);
+    const float dl = fmaxf(0.0f, fabsf(lc) - hl), de = fmaxf(0.0f, fabsf(ec) - he);
+    return !(cp.g_a * dl * dl + cp.g_b * de * de >= cp.g_max);
+}
+
 // Can a point of box B have a non-zero weight and a bin inside [r_lo, r_hi) on
 // line L?  Conservative: bounds are padded by a relative margin that covers the
 // float rounding of the per-point evaluation.
@@ -64,25 +79,21 @@ __device__ __forceinline__ bool chunk_hits(const CullParams& cp, float r_lo, flo
         hit_r = (rc + hr > r_lo) && (rc - hr < r_hi);
     }
     if (!hit_r) return false;
-    const float lc = dx * L.lx + dy * L.ly + dz * L.lz;
-    const float hl = B.hx * fabsf(L.lx) + B.hy * fabsf(L.ly) + B.hz * fabsf(L.lz) + m;
-    const float ec = dx * L.ex + dy * L.ey + dz * L.ez;
-    const float he = B.hx * fabsf(L.ex) + B.hy * fabsf(L.ey) + B.hz * fabsf(L.ez) + m;
-    if (LUT) return (lc + hl > cp.l_lo) && (lc - hl < cp.l_hi) && (ec + he > cp.e_lo) && (ec - he < cp.e_hi);
-    const float dl = fmaxf(0.0f, fabsf(lc) - hl), de = fmaxf(0.0f, fabsf(ec) - he);
-    return !(cp.g_a * dl * dl + cp.g_b * de * de >= cp.g_max);
+    return lateral_hits<LUT>(cp, L, B, dx, dy, dz, m);
 }
 
-// Radial-only test of a box against the window for lines from a common origin.
-__device__ __forceinline__ bool chunk_in_window(float r_lo, float r_hi, const LineGeom& L, const ChunkBox& B) {
-    if (!(B.hx >= 0.0f)) return false;
-    const float dx = B.cx - L.ox, dy = B.cy - L.oy, dz = B.cz - L.oz;
-    const float hn = sqrtf(B.hx * B.hx + B.hy * B.hy + B.hz * B.hz);
-    const float dn = sqrtf(dx * dx + dy * dy + dz * dz);
-    const float m = 1e-6f * (fabsf(B.cx) + fabsf(B.cy) + fabsf(B.cz) + fabsf(L.ox) + fabsf(L.oy) + fabsf(L.oz) +
-                             dn + hn) + 1e-30f;
-    const float rmin = fmaxf(0.0f, dn - hn) - m, rmax = dn + hn + m;
-    return (rmax > r_lo && rmin < r_hi) || (-rmin > r_lo && -rmax < r_hi);
+// chunk_hits for lines from a common origin with arc projection, the distance
+// terms of the box (computed once per chunk by the caller) given: dx, dy, dz the
+// box centre from the origin, m the margin, pos_ok / neg_ok whether the box's
+// distance range meets the window on the positive / negative side.
+template <bool LUT>
+__device__ __forceinline__ bool chunk_hits_shared(const CullParams& cp, const LineGeom& L, const ChunkBox& B,
+                                                  float dx, float dy, float dz, float m, bool pos_ok,
+                                                  bool neg_ok) {
+    const float rc = dx * L.rx + dy * L.ry + dz * L.rz;
+    const float hr = B.hx * fabsf(L.rx) + B.hy * fabsf(L.ry) + B.hz * fabsf(L.rz) + m;
+    if (!((pos_ok && rc + hr > 0.0f) || (neg_ok && rc - hr < 0.0f))) return false;
+    return lateral_hits<LUT>(cp, L, B, dx, dy, dz, m);
 }
 
 // Weight times amplitude added at window-relative bin lb (CpuAlgorithm.cpp:77-95).
@@ -206,10 +217,30 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
         if ((int)threadIdx.x < step && pos < cend) {
             const int ci = g.chunks ? g.chunks[pos] : pos;
             const ChunkBox B = boxes[ci];
-            if (!(SHARED && ARC) || chunk_in_window(r_lo, r_hi, tl[0], B)) {
-                unsigned mask = 0;
+            unsigned mask = 0;
+            if (SHARED && ARC) {
+                // lines from a common origin: the box's distance range once per chunk.
+                // The hardware square root (within an ulp or two) is covered by the
+                // margin m, a relative 1e-6.
+                if (B.hx >= 0.0f) {
+                    const float dx = B.cx - org_x, dy = B.cy - org_y, dz = B.cz - org_z;
+                    const float hn = __builtin_amdgcn_sqrtf(B.hx * B.hx + B.hy * B.hy + B.hz * B.hz);
+                    const float dn = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+                    const float m = 1e-6f * (fabsf(B.cx) + fabsf(B.cy) + fabsf(B.cz) + fabsf(org_x) + fabsf(org_y) +
+                                             fabsf(org_z) + dn + hn) + 1e-30f;
+                    const float rmin = fmaxf(0.0f, dn - hn) - m, rmax = dn + hn + m;
+                    const bool pos_ok = rmax > r_lo && rmin < r_hi;
+                    const bool neg_ok = -rmin > r_lo && -rmax < r_hi;
+                    if (pos_ok || neg_ok) {
+                        for (int k = 0; k < nl; k++)
+                            if (chunk_hits_shared<LUT>(cp, tl[k], B, dx, dy, dz, m, pos_ok, neg_ok)) mask |= 1u << k;
+                    }
+                }
+            } else {
                 for (int k = 0; k < nl; k++)
                     if (chunk_hits<ARC, LUT>(cp, r_lo, r_hi, tl[k], B)) mask |= 1u << k;
+            }
+            {
                 if (mask) {
                     const int slot = atomicAdd(&list_n, 1);
                     list_chunk[slot] = ci;
