@@ -330,3 +330,27 @@ def test_tile_order_keeps_frames():
     err = rel_l2(iq[pick], o.simulate_frame())
     print(f"[result] ordered frame vs oracle (4 lines) rel_l2={err:.2e}", flush=True)
     assert err <= TOL
+
+
+def test_tile_order_many_blocks_and_windows():
+    """Block ordering over thousands of blocks (one line per tile) and over
+    (tile, window) blocks: frames equal those of the index order."""
+    from openbcsim_amd.pyrfsim import RfSimulator
+    wl = W.volume_lut(200_000, n_az=64, n_el=64, seed=44)
+    res = {}
+    for mode in ("off", "on"):
+        s = RfSimulator("gpu")
+        wl.apply(s)
+        s.set_parameter("tile_order", mode)
+        out = []
+        for lines, windows in ((1, 1), (2, 2), (4, 1)):
+            s.set_parameter("tile_lines", str(lines))
+            s.set_parameter("tile_windows", str(windows))
+            out.append([s.simulate_frame() for _ in range(2)])
+        res[mode] = out
+        s.close()
+    for k, (lines, windows) in enumerate(((1, 1), (2, 2), (4, 1))):
+        for f in range(2):
+            e = rel_l2(res["on"][k][f], res["off"][k][f])
+            print(f"[result] tile_lines={lines} windows={windows} frame {f}: order on vs off {e:.2e}", flush=True)
+            assert e < 1e-6
