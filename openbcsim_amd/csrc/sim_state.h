@@ -330,7 +330,7 @@ struct bcsim_sim {
     // profile
     int profile = -1;
     float sigma_l = 0.f, sigma_e = 0.f;
-    std::shared_ptr<DevBuf> lut;  // float4 quads [n_rad][n_lat][n_ele]; replaced, never rewritten
+    std::shared_ptr<DevBuf> lut;  // float4 quad pairs [n_rad][n_lat][n_ele][2]; replaced, never rewritten
     int lut_nr = 0, lut_nl = 0, lut_ne = 0;
     float lut_r0 = 0, lut_r1 = 0, lut_l0 = 0, lut_l1 = 0, lut_e0 = 0, lut_e1 = 0;
 
