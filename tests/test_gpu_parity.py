@@ -354,3 +354,25 @@ def test_tile_order_many_blocks_and_windows():
             e = rel_l2(res["on"][k][f], res["off"][k][f])
             print(f"[result] tile_lines={lines} windows={windows} frame {f}: order on vs off {e:.2e}", flush=True)
             assert e < 1e-6
+
+
+@pytest.mark.parametrize("phase", [False, True])
+def test_smallest_transform_every_decimation(phase):
+    """A 5-sample excitation and a 0.18 mm line give the smallest transform
+    (16 points): the real-input path then runs an 8-point forward transform in
+    one pass, and no decimation can fold the inverse below 16 points."""
+    rng = np.random.default_rng(45)
+    for dec in (1, 2, 3, 4, 8, 16):
+        wl = W.fixed_sector_2d(1, 6, line_length=0.00018, decimation=dec, phase_delay=phase)
+        wl.excitation = np.array([0.2, -0.7, 1.0, -0.6, 0.25], np.float32)
+        wl.center_index = 2
+        pts = np.zeros((400, 4), np.float32)
+        pts[:, 0] = rng.uniform(-0.0005, 0.0005, 400)
+        pts[:, 2] = rng.uniform(0.00002, 0.00017, 400)
+        pts[:, 3] = rng.standard_normal(400)
+        wl.fixed = [pts]
+        assert wl.fft_len() == 16, wl.fft_len()
+        g, iq, bins, o = run_pair(wl)
+        err = rel_l2(iq, o.simulate_frame())
+        print(f"[result] N_fft=16 decimation={dec} phase={phase}: {iq.shape} rel_l2={err:.2e}", flush=True)
+        assert err <= TOL
