@@ -26,6 +26,7 @@
 // Each request has its own bins and tables, so its result does not depend on
 // what it is batched with, up to float summation order.
 #include "batch_state.h"
+#include "host_copy.h"
 
 #include <chrono>
 #include <condition_variable>
@@ -99,26 +100,6 @@ class PinnedPool {
     std::map<size_t, std::vector<void*>> free_;
     size_t pooled_ = 0;
 };
-
-// memcpy split over a few threads for large copies (page faults of a fresh
-// destination are taken in parallel too).
-void parallel_copy(void* dst, const void* src, size_t bytes) {
-    constexpr size_t kChunk = (size_t)4 << 20;
-    const int parts = (int)std::min<size_t>(8, bytes / kChunk);
-    if (parts <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t step = (bytes + parts - 1) / parts;
-    std::vector<std::thread> th;
-    for (int i = 1; i < parts; i++) {
-        const size_t off = (size_t)i * step;
-        const size_t n = std::min(step, bytes - std::min(bytes, off));
-        th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, n); });
-    }
-    std::memcpy(dst, src, std::min(step, bytes));
-    for (auto& t : th) t.join();
-}
 
 struct Request {
     uint64_t id = 0;

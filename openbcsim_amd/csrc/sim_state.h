@@ -19,6 +19,7 @@
 
 #include "../../include/bcsim_c.h"
 #include "autotune.h"
+#include "host_copy.h"
 #include "sim_device.h"
 
 
@@ -346,6 +347,10 @@ struct bcsim_sim {
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
+        }
+        for (int i = 0; i < 2; i++) {
+            if (stage[i]) (void)hipHostFree(stage[i]);
+            if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
         }
     }
 
@@ -1440,16 +1445,53 @@ struct bcsim_sim {
         if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
         d_iq.ensure(need * sizeof(float2));
         enqueue_frame(d_iq.as<float2>());
-        {
-            Timed t(this, "kernel_memcpy_ms");
-            hip_check(hipMemcpyAsync(out, d_iq.p, need * sizeof(float2), hipMemcpyDeviceToHost, stream),
-                      "hipMemcpyAsync");
-            t.stop();
+        const size_t bytes = need * sizeof(float2);
+        if (bytes >= kStagedCopyMin && host_pageable(out)) {
+            copy_staged(out, d_iq.p, bytes);
+        } else {
+            {
+                Timed t(this, "kernel_memcpy_ms");
+                hip_check(hipMemcpyAsync(out, d_iq.p, bytes, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+                t.stop();
+            }
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         }
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         if (store_details) collect_timings();
         if (nl) *nl = (int)lines.size();
         if (ns) *ns = n_out;
+    }
+
+    // Frames copied into pageable host memory pass through two pinned pieces of
+    // kStagePiece bytes: the DMA of piece c + 1 runs while host threads move piece
+    // c into the caller's buffer (the runtime's own path to pageable memory is
+    // about a third of the DMA rate).
+    static constexpr size_t kStagedCopyMin = (size_t)8 << 20;
+    static constexpr size_t kStagePiece = (size_t)32 << 20;
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+
+    void copy_staged(void* dst, const void* src, size_t bytes) {
+        for (int i = 0; i < 2; i++) {
+            if (!stage[i]) hip_check(hipHostMalloc(&stage[i], kStagePiece, 0), "hipHostMalloc");
+            if (!stage_ev[i]) hip_check(hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming), "hipEventCreate");
+        }
+        const size_t pieces = (bytes + kStagePiece - 1) / kStagePiece;
+        auto piece_len = [&](size_t c) { return std::min(kStagePiece, bytes - c * kStagePiece); };
+        auto issue = [&](size_t c) {
+            hip_check(hipMemcpyAsync(stage[c % 2], (const char*)src + c * kStagePiece, piece_len(c),
+                                     hipMemcpyDeviceToHost, stream),
+                      "hipMemcpyAsync");
+            hip_check(hipEventRecord(stage_ev[c % 2], stream), "hipEventRecord");
+        };
+        Timed t(this, "kernel_memcpy_ms");
+        issue(0);
+        for (size_t c = 0; c < pieces; c++) {
+            if (c + 1 < pieces) issue(c + 1);  // its pinned piece was emptied in the previous round
+            if (c + 1 == pieces) t.stop();
+            hip_check(hipEventSynchronize(stage_ev[c % 2]), "hipEventSynchronize");
+            parallel_copy((char*)dst + c * kStagePiece, stage[c % 2], piece_len(c));
+        }
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
     void debug_projection(float* out, size_t capacity, int* nl, int* nrf) {

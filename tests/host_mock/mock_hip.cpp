@@ -235,6 +235,18 @@ hipError_t hipHostFree(void* ptr) {
     return hipSuccess;
 }
 
+hipError_t hipPointerGetAttributes(hipPointerAttribute_t* attr, const void* ptr) {
+    std::lock_guard<std::mutex> g(g_alloc_mu);
+    const uintptr_t p = (uintptr_t)ptr;
+    auto it = g_allocs.upper_bound(p);
+    if (it == g_allocs.begin()) return hipErrorInvalidValue;
+    --it;
+    if (p >= it->first + std::max<size_t>(it->second.size, 1)) return hipErrorInvalidValue;
+    *attr = hipPointerAttribute_t{};
+    attr->type = it->second.pinned ? hipMemoryTypeHost : hipMemoryTypeDevice;
+    return hipSuccess;
+}
+
 hipError_t hipMemcpy(void* dst, const void* src, size_t n, hipMemcpyKind kind) {
     const bool dd = kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice;
     const bool sd = kind == hipMemcpyDeviceToHost || kind == hipMemcpyDeviceToDevice;
