@@ -188,7 +188,7 @@ BCSIM_API bcsim_status bcsim_batch_destroy(bcsim_batch* batch);
  * batch_frame_ms with HIP events, waiting for each frame), autotune (on/off,
  * default on: each job's shape class is tuned before its first batched frame),
  * tile_lines (lines per tile for every job, 0 = each job's plan),
- * items_per_workgroup (projection work items per persistent workgroup, default 32). */
+ * items_per_workgroup (projection work items per persistent workgroup, default 4). */
 BCSIM_API bcsim_status bcsim_batch_set_parameter(bcsim_batch* batch, const char* key, const char* value);
 BCSIM_API bcsim_status bcsim_batch_add(bcsim_batch* batch, bcsim_sim* sim, int* job_index);
 BCSIM_API bcsim_status bcsim_batch_clear(bcsim_batch* batch);
@@ -213,9 +213,12 @@ BCSIM_API bcsim_status bcsim_batch_get_debug_data(bcsim_batch* batch, const char
  * (phantom, scan geometry, time t) from any thread.  A scheduler thread admits
  * requests in arrival order while their frame buffers fit the memory budget and
  * launches every admitted request as one batched frame; the next batch is formed
- * and launched while the previous one runs (two frames in flight).  A request's
- * IQ equals what a simulator of that phantom with that scan (all timestamps t)
- * returns from simulate_lines, up to float summation order.
+ * and launched while the previous one runs (frame_contexts frames in flight).
+ * A request of more than piece_lines lines runs as pieces (consecutive line
+ * ranges) in successive frames, so that its first lines cross PCIe while the
+ * rest compute.  A request's IQ equals what a simulator of that phantom with
+ * that scan (all timestamps t) returns from simulate_lines, up to float
+ * summation order.
  */
 typedef struct bcsim_server bcsim_server;
 
@@ -225,7 +228,9 @@ BCSIM_API bcsim_status bcsim_server_destroy(bcsim_server* server);
 /* Keys: memory_budget_bytes (device bytes for phantoms, frame buffers and
  * outputs; default 90% of free device memory when the first phantom is added),
  * max_jobs_per_frame (default 64), autotune (on/off, default on),
- * reset_stats (any value: clears the statistics). */
+ * piece_lines (default 1024; 0: never split a request), frame_contexts
+ * (frames in flight, 1-4, default 2), reset_stats (any value: clears the
+ * statistics). */
 BCSIM_API bcsim_status bcsim_server_set_parameter(bcsim_server* server, const char* key, const char* value);
 /* Snapshot of a configured simulator (excitation, parameters, profile,
  * scatterer datasets): its datasets stay resident, shared with the caller's
@@ -253,7 +258,8 @@ BCSIM_API bcsim_status bcsim_server_wait(bcsim_server* server, uint64_t request_
                                          int* num_lines, int* num_samples, double* latency_ms);
 /* Keys: latency_ms, queue_ms (submit to launch), copy_ms (the copy in wait)
  * per waited request;
- * frame_jobs, frame_lines per launched frame; lines_completed, busy_span_ms
+ * frame_jobs, frame_lines, launch_ms (host time to prepare and enqueue) per
+ * launched frame; lines_completed, busy_span_ms
  * (first submit to last completion), budget_bytes, used_bytes, resident_bytes,
  * idle_slots, pending (one value each).  out NULL: count only. */
 BCSIM_API bcsim_status bcsim_server_get_stats(bcsim_server* server, const char* key, double* out,

@@ -9,8 +9,10 @@
 //   1. zeroes the bins of jobs that accumulate and resets the work counters
 //      (one fill kernel, no memset nodes, so the sequence can be captured),
 //   2. renders all-timestamps-equal spline sets,
-//   3. runs one persistent projection kernel over (job, tile-block) items,
-//      heaviest items first,
+//   3. orders each job's blocks by the work they listed in the previous frame
+//      (one order kernel, a workgroup per job) and runs one persistent
+//      projection kernel over (job, tile-block) items, heaviest jobs first and
+//      within a job its heaviest blocks first,
 //   4. runs one persistent convolution kernel over (job, line) items, longest
 //      transforms first, writing each job's IQ at its offset of the output.
 // The sequence is captured once into a HIP graph and replayed while no job's
@@ -19,6 +21,7 @@
 #include "sim_state.h"
 
 #include <numeric>
+#include <tuple>
 
 namespace bcsim_hip {
 
@@ -42,8 +45,8 @@ struct bcsim_batch {
     bool planned = false;
     std::vector<JobPlan> plan;
     size_t total_out = 0;
-    DevBuf d_proj_jobs, d_proj_items, d_conv_jobs, d_conv_items, d_fill, d_counters;
-    int n_proj_items = 0, n_conv_items = 0, n_fill = 0;
+    DevBuf d_proj_jobs, d_proj_items, d_conv_jobs, d_conv_items, d_fill, d_counters, d_order_segs;
+    int n_proj_items = 0, n_conv_items = 0, n_fill = 0, n_order_segs = 0;
     size_t proj_lds = 0;
     struct Render {
         SplineSet set;
@@ -55,7 +58,9 @@ struct bcsim_batch {
     bool have_cu_count = false;
     bool autotune = true;          // tune each job's shape class before its first batched frame
     int tile_lines = 0;            // lines per tile for every job; 0 = each job's plan
-    int items_per_workgroup = 32;  // projection items per persistent workgroup (balance vs. overhead)
+    int items_per_workgroup = 4;   // projection items per persistent workgroup (balance vs. overhead)
+    // Jobs of fewer blocks keep the index order (a segment of the order kernel each otherwise).
+    static constexpr int kMinOrderedJobBlocks = 16;
 
     // graph of the last capture
     hipGraph_t graph = nullptr;
@@ -70,7 +75,55 @@ struct bcsim_batch {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
+        if (h_stage) (void)hipHostFree(h_stage);
     }
+
+    // Device tables grow by at least half their size, so a batch whose mix of jobs
+    // changes from frame to frame (the server's) soon stops reallocating: a
+    // hipFree waits for every stream of the device.
+    static void grow(DevBuf& d, size_t bytes) {
+        if (bytes <= d.bytes && d.p) return;
+        d.ensure(std::max<size_t>({bytes, d.bytes + d.bytes / 2, 256}));
+    }
+
+    // Pinned host staging of the tables, copied on `stream` without the host
+    // waiting: a synchronous copy would wait for other streams' kernels that hold
+    // the CUs.  Reused once the stream has drained (build() and set_output()
+    // synchronise it first); the head [0, head) is kept for set_output's table.
+    void* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    bool just_built = false;
+    void ensure_stage(size_t bytes) {
+        if (bytes <= h_stage_bytes) return;
+        if (h_stage) (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        h_stage_bytes = 0;
+        const size_t n = std::max<size_t>(bytes, 64 << 10) * 2;
+        hip_check(hipHostMalloc(&h_stage, n, 0), "hipHostMalloc");
+        h_stage_bytes = n;
+    }
+    struct Stager {
+        bcsim_batch& b;
+        size_t head;
+        std::vector<std::tuple<DevBuf*, const void*, size_t, size_t>> parts;  // device, source, bytes, offset
+        size_t end;
+        Stager(bcsim_batch& b_, size_t head_bytes) : b(b_), head((head_bytes + 255) & ~(size_t)255), end(head) {}
+        void add(DevBuf& d, const void* src, size_t bytes) {
+            parts.emplace_back(&d, src, bytes, end);
+            end += (bytes + 255) & ~(size_t)255;
+        }
+        void upload() {
+            b.ensure_stage(end);
+            char* h = static_cast<char*>(b.h_stage);
+            for (auto& p : parts) {
+                const size_t bytes = std::get<2>(p);
+                if (!bytes) continue;
+                std::memcpy(h + std::get<3>(p), std::get<1>(p), bytes);
+                hip_check(hipMemcpyAsync(std::get<0>(p)->p, h + std::get<3>(p), bytes, hipMemcpyHostToDevice, b.stream),
+                          "hipMemcpyAsync");
+            }
+        }
+    };
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -138,6 +191,7 @@ struct bcsim_batch {
         std::vector<double> pcost;  // per-item cost estimate of each projection job
         std::vector<ConvParams> cjobs;
         std::vector<FillSeg> fills;
+        std::vector<std::pair<int, int*>> ordered;  // (projection job, its work counts)
         std::vector<int> job_first_target, job_targets;
         size_t offset = 0;
         proj_lds = 0;
@@ -151,7 +205,8 @@ struct bcsim_batch {
             const bool shared = s->shared_origin();
             const std::vector<bcsim_sim::ProjTarget> targets = s->proj_targets();
             const bool tune_now = autotune || s->autotune;
-            for (const auto& t : targets) {
+            for (size_t ti = 0; ti < targets.size(); ti++) {
+                const auto& t = targets[ti];
                 const bool is_spline = std::strcmp(t.kind, "spline") == 0;
                 ProjJob J{};
                 J.c = t.c;
@@ -173,6 +228,16 @@ struct bcsim_batch {
                           (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0);
                 const int tiles = (t.c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
                 J.blocks_per_slice = tiles * J.g.windows;
+                J.g.work = nullptr;
+                J.g.order = nullptr;
+                if (s->tile_order && J.blocks_per_slice >= kMinOrderedJobBlocks &&
+                    J.blocks_per_slice <= kMaxOrderedTiles) {
+                    // the job's blocks in the order of the work they listed in the previous
+                    // frame, heaviest first (counts in the simulator's own buffer: zeroed on
+                    // its stream, which quiesce() below waits for)
+                    J.g.work = s->order_buffer(ti, J.blocks_per_slice);
+                    ordered.push_back({(int)pjobs.size(), J.g.work});
+                }
                 pjobs.push_back(J);
                 pcost.push_back((double)t.nchunks * t.c.num_lines * (is_spline ? 3.0 : 1.0));
                 if (std::strcmp(t.kind, "rendered") == 0) {
@@ -208,7 +273,9 @@ struct bcsim_batch {
         std::iota(porder.begin(), porder.end(), 0);
         std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return pcost[a] > pcost[b]; });
         std::vector<int2> pitems;
+        std::vector<size_t> first_item(pjobs.size(), 0);
         for (int j : porder) {
+            first_item[j] = pitems.size();
             const int n = pjobs[j].blocks_per_slice * pjobs[j].g.slices;
             for (int i = 0; i < n; i++) pitems.push_back(make_int2(j, i));
         }
@@ -220,21 +287,33 @@ struct bcsim_batch {
         for (int j : corder)
             for (int l = 0; l < cjobs[j].num_lines; l++) citems.push_back(make_int2(j, l));
 
-        auto upload = [&](DevBuf& d, const void* src, size_t bytes) {
-            d.ensure(std::max<size_t>(bytes, 16));
-            if (bytes) hip_check(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
-        };
-        upload(d_proj_jobs, pjobs.data(), pjobs.size() * sizeof(ProjJob));
-        upload(d_proj_items, pitems.data(), pitems.size() * sizeof(int2));
-        upload(d_fill, fills.data(), fills.size() * sizeof(FillSeg));
-        upload(d_conv_items, citems.data(), citems.size() * sizeof(int2));
-        d_conv_jobs.ensure(std::max<size_t>(cjobs.size() * sizeof(ConvParams), 16));
-        d_counters.ensure(2 * sizeof(int));
+        grow(d_proj_jobs, pjobs.size() * sizeof(ProjJob));
+        grow(d_proj_items, pitems.size() * sizeof(int2));
+        grow(d_fill, fills.size() * sizeof(FillSeg));
+        grow(d_order_segs, ordered.size() * sizeof(OrderSeg));
+        grow(d_conv_items, citems.size() * sizeof(int2));
+        grow(d_conv_jobs, cjobs.size() * sizeof(ConvParams));
+        grow(d_counters, 2 * sizeof(int));
+        std::vector<OrderSeg> order_segs;
+        for (const auto& o : ordered) {
+            const ProjJob& J = pjobs[o.first];
+            order_segs.push_back(OrderSeg{o.second, d_proj_items.as<int2>() + first_item[o.first], J.blocks_per_slice,
+                                          J.g.slices});
+        }
+        Stager st(*this, cjobs.size() * sizeof(ConvParams));
+        st.add(d_proj_jobs, pjobs.data(), pjobs.size() * sizeof(ProjJob));
+        st.add(d_proj_items, pitems.data(), pitems.size() * sizeof(int2));
+        st.add(d_fill, fills.data(), fills.size() * sizeof(FillSeg));
+        st.add(d_order_segs, order_segs.data(), order_segs.size() * sizeof(OrderSeg));
+        st.add(d_conv_items, citems.data(), citems.size() * sizeof(int2));
+        st.upload();
         conv_host = cjobs;
         n_proj_items = (int)pitems.size();
         n_conv_items = (int)citems.size();
         n_fill = (int)fills.size();
+        n_order_segs = (int)order_segs.size();
         planned = true;
+        just_built = true;
     }
     std::vector<ConvParams> conv_host;
 
@@ -243,11 +322,20 @@ struct bcsim_batch {
     float2* table_out = nullptr;
     void set_output(float2* out) {
         if (table_out == out) return;
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        std::vector<ConvParams> cj = conv_host;
-        for (size_t i = 0; i < cj.size(); i++) cj[i].iq = out + plan[i].out_offset;
-        hip_check(hipMemcpy(d_conv_jobs.p, cj.data(), cj.size() * sizeof(ConvParams), hipMemcpyHostToDevice),
-                  "hipMemcpy");
+        // frames in flight read the table, and an earlier upload may still read the
+        // staging head; right after build() neither holds (it drained the stream and
+        // its own uploads use the rest of the staging buffer)
+        if (!just_built) hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        just_built = false;
+        // the head of the staging buffer (build() keeps it for this table)
+        const size_t bytes = conv_host.size() * sizeof(ConvParams);
+        ensure_stage(bytes);
+        ConvParams* cj = static_cast<ConvParams*>(h_stage);
+        for (size_t i = 0; i < conv_host.size(); i++) {
+            cj[i] = conv_host[i];
+            cj[i].iq = out + plan[i].out_offset;
+        }
+        if (bytes) hip_check(hipMemcpyAsync(d_conv_jobs.p, cj, bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
         table_out = out;
     }
 
@@ -255,6 +343,7 @@ struct bcsim_batch {
         hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), 2, stream), "fill kernel");
         for (const Render& r : renders)
             hip_check(launch_render_spline(r.set, r.basis, r.out, stream), "render kernel");
+        hip_check(launch_order_segments(d_order_segs.as<OrderSeg>(), n_order_segs, stream), "block order kernel");
         hip_check(launch_project_batch(d_proj_jobs.as<ProjJob>(), d_proj_items.as<int2>(), n_proj_items,
                                        d_counters.as<int>(), proj_lds, workgroups, stream),
                   "batched projection kernel");

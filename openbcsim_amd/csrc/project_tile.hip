@@ -381,7 +381,9 @@ __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullPa
 // instead of a full sort (order within a bucket is arbitrary).
 constexpr int kOrderBuckets = 256;
 
-__global__ __launch_bounds__(1024) void order_tiles_kernel(int* __restrict__ work, int* __restrict__ order, int n) {
+// place(i, r): block i goes to position r of the order.
+template <class Place>
+__device__ __forceinline__ void order_blocks(int* __restrict__ work, int n, Place place) {
     __shared__ int count[kOrderBuckets];
     __shared__ int start[kOrderBuckets];
     __shared__ int wmax;
@@ -422,14 +424,34 @@ __global__ __launch_bounds__(1024) void order_tiles_kernel(int* __restrict__ wor
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int w = work[i];
-        order[atomicAdd(&start[bucket(w)], 1)] = i;
+        place(i, atomicAdd(&start[bucket(w)], 1));
         work[i] = 0;
     }
+}
+
+__global__ __launch_bounds__(1024) void order_tiles_kernel(int* __restrict__ work, int* __restrict__ order, int n) {
+    order_blocks(work, n, [order](int i, int r) { order[r] = i; });
+}
+
+// A batched job's items are (job, slice * n + block), slice-major; the order is
+// written into them (the block of item slice * n + r is the r-th heaviest), so
+// the persistent kernel takes the job's heaviest blocks first of every slice.
+__global__ __launch_bounds__(1024) void order_segments_kernel(const OrderSeg* __restrict__ segs) {
+    const OrderSeg S = segs[blockIdx.x];
+    order_blocks(S.work, S.n, [S](int i, int r) {
+        for (int s = 0; s < S.slices; s++) S.items[(size_t)s * S.n + r].y = s * S.n + i;
+    });
 }
 
 hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream) {
     if (n <= 0 || n > kMaxOrderedTiles) return hipErrorInvalidValue;
     hipLaunchKernelGGL(order_tiles_kernel, dim3(1), dim3(1024), 0, stream, work, order, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t stream) {
+    if (n_segs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(order_segments_kernel, dim3(n_segs), dim3(1024), 0, stream, segs);
     return hipGetLastError();
 }
 

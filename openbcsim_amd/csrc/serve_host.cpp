@@ -12,8 +12,13 @@
 //     for the next request of the same phantom and geometry, or evicted least
 //     recently used first), and launches the admitted requests as one batched
 //     frame (one persistent projection launch, one persistent convolution
-//     launch) on one of kFrameContexts frame streams, so the next batches are
+//     launch) on one of `contexts` frame streams, so the next batches are
 //     formed and launched while earlier ones compute or copy;
+//   - a request of more than `piece_lines` lines is served in pieces
+//     (consecutive line ranges), at most one piece of a request per frame, and
+//     frames holding pieces run their kernels one after another: the IQ of one
+//     piece crosses PCIe while the next piece computes, instead of the whole
+//     frame's copy starting after its last kernel;
 //   - at the end of its frame each request's IQ is copied by DMA, smallest
 //     first, straight into the caller's buffer when that is pinned memory of the
 //     server's (bcsim_server_alloc_host), otherwise into a pinned staging
@@ -46,6 +51,10 @@ struct Geometry {
     std::vector<float> o, d, lat, t;
     float length = 0.f;
     int lines() const { return (int)t.size(); }
+    // consecutive line ranges of at most `piece_size` lines, each an internal
+    // geometry (ids of bcsim_server::piece_geoms) served as a job of its own
+    int piece_size = 0;
+    std::vector<int> pieces;
 };
 
 // Frame buffers of one in-flight or idle request: a simulator sharing its
@@ -116,6 +125,12 @@ struct Request {
     void* staged = nullptr;  // pinned copy of the IQ
     size_t staged_cls = 0;
     bool direct = false;     // `out` is pinned memory of the server's: the IQ is copied there by DMA
+    // A request split into pieces (line ranges) owns them and is done when the
+    // last one is; a piece writes lines [line0, line0 + lines) of its parent's IQ.
+    Request* parent = nullptr;
+    int line0 = 0;
+    std::vector<std::unique_ptr<Request>> pieces;
+    int pieces_left = 0;
 };
 
 struct Shape {
@@ -132,11 +147,14 @@ struct Frame {
     DevBuf d_out;
     size_t out_accounted = 0;  // bytes of d_out counted in the server's `used`
     std::vector<hipEvent_t> events;  // reused; events[k] follows the copy of copy_order[k]
+    hipEvent_t computed = nullptr;   // after the frame's kernels, before its copies
+    bool has_piece = false;
     std::vector<Request*> reqs;      // in batch (job) order
     std::vector<Request*> copy_order;
     bool busy = false;
     ~Frame() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (computed) (void)hipEventDestroy(computed);
     }
 };
 
@@ -150,6 +168,9 @@ struct bcsim_server {
     size_t resident = 0;     // phantom datasets and LUTs (part of `used`)
     int max_jobs = 64;       // requests per batched frame
     bool autotune = true;
+    int piece_lines = 1024;  // requests of more lines are served in pieces of this many (0: never)
+    static constexpr int kMaxFrameContexts = 4;
+    int contexts = 2;        // frames in flight
 
     std::mutex mu;
     std::condition_variable cv_sched, cv_comp, cv_done;
@@ -164,16 +185,17 @@ struct bcsim_server {
     std::vector<std::unique_ptr<bcsim_sim>> phantoms;  // snapshots (views) of the callers' simulators
     std::set<const void*> resident_sets;               // datasets and LUTs already accounted
     std::vector<Geometry> geometries;
+    std::deque<Geometry> piece_geoms;  // internal geometries of pieces: id -(k + 1) is piece_geoms[k]
     std::map<std::pair<int, int>, Shape> shapes;
     std::vector<std::unique_ptr<Slot>> slots;
     std::deque<Request*> pending;
     std::map<uint64_t, std::unique_ptr<Request>> requests;
-    static constexpr int kFrameContexts = 2;
-    Frame frames[kFrameContexts];
+    Frame frames[kMaxFrameContexts];
     std::deque<Frame*> inflight;
+    Frame* last_piece_frame = nullptr;  // the last frame launched with a piece in it
 
     // statistics (completion order)
-    std::vector<double> lat_ms, queue_ms, copy_ms, frame_jobs, frame_lines;
+    std::vector<double> lat_ms, queue_ms, copy_ms, frame_jobs, frame_lines, launch_ms;
     double lines_done = 0;
     double t_first_submit = -1, t_last_done = -1;
 
@@ -250,20 +272,46 @@ struct bcsim_server {
         return (int)geometries.size() - 1;
     }
 
+    // A caller's geometry (id >= 0) or a piece's (id < 0).
+    const Geometry& geom(int id) const { return id >= 0 ? geometries[id] : piece_geoms[-id - 1]; }
+
+    // The pieces of geometry gi for the current piece size, made on first use (mu held).
+    const std::vector<int>& pieces_of(int gi) {
+        if (geometries[gi].piece_size != piece_lines) {
+            const Geometry src = geometries[gi];
+            std::vector<int> ids;
+            for (int a = 0; a < src.lines(); a += piece_lines) {
+                const int n = std::min(piece_lines, src.lines() - a);
+                Geometry pg;
+                pg.o.assign(src.o.begin() + 3 * (size_t)a, src.o.begin() + 3 * (size_t)(a + n));
+                pg.d.assign(src.d.begin() + 3 * (size_t)a, src.d.begin() + 3 * (size_t)(a + n));
+                pg.lat.assign(src.lat.begin() + 3 * (size_t)a, src.lat.begin() + 3 * (size_t)(a + n));
+                pg.t.assign(n, 0.0f);
+                pg.length = src.length;
+                piece_geoms.push_back(std::move(pg));
+                ids.push_back(-(int)piece_geoms.size());
+            }
+            geometries[gi].pieces = std::move(ids);
+            geometries[gi].piece_size = piece_lines;
+        }
+        return geometries[gi].pieces;
+    }
+
     // Simulator for (phantom, geometry) with its frame shape known (nothing allocated).
     std::unique_ptr<bcsim_sim> make_sim(int p, int gi) {
         auto s = std::make_unique<bcsim_sim>();
         s->configure_view_of(*phantoms[p]);
-        const Geometry& gm = geometries[gi];
+        const Geometry& gm = geom(gi);
         s->set_scan_sequence(gm.lines(), gm.o.data(), gm.d.data(), gm.lat.data(), gm.t.data(), gm.length, true);
         s->update_shape();
         return s;
     }
 
-    // Output shape and frame-buffer estimate of a pair (mu held).
+    // Output shape and frame-buffer estimate of a pair (mu held); gi may be a piece's.
     const Shape& shape_of(int p, int gi) {
         if (p < 0 || p >= (int)phantoms.size()) fail(BCSIM_ERR_INVALID_ARGUMENT, "unknown phantom");
-        if (gi < 0 || gi >= (int)geometries.size()) fail(BCSIM_ERR_INVALID_ARGUMENT, "unknown geometry");
+        if (gi >= (int)geometries.size() || gi < -(int)piece_geoms.size())
+            fail(BCSIM_ERR_INVALID_ARGUMENT, "unknown geometry");
         auto key = std::make_pair(p, gi);
         auto it = shapes.find(key);
         if (it != shapes.end()) return it->second;
@@ -275,13 +323,19 @@ struct bcsim_server {
         return shapes[key] = sh;
     }
 
+    // shape_of for a caller's geometry id (mu held).
+    const Shape& user_shape(int p, int gi) {
+        if (gi < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "unknown geometry");
+        return shape_of(p, gi);
+    }
+
     // ------------------------------------------------------------ requests
     uint64_t submit(int p, int gi, float t, float* out, size_t capacity) {
         if (!out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output buffer");
         if (!std::isfinite(t)) fail(BCSIM_ERR_INVALID_ARGUMENT, "time must be finite");
         std::lock_guard<std::mutex> g(mu);
         if (stop) fail(BCSIM_ERR_INVALID_STATE, "server is shutting down");
-        const Shape& sh = shape_of(p, gi);
+        const Shape& sh = user_shape(p, gi);
         if (capacity < (size_t)sh.lines * sh.samples) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
         auto r = std::make_unique<Request>();
         r->id = next_id++;
@@ -295,7 +349,30 @@ struct bcsim_server {
         r->direct = in_host_block(out, (size_t)sh.lines * sh.samples * sizeof(float2));
         r->t_submit = now_ms();
         if (t_first_submit < 0) t_first_submit = r->t_submit;
-        pending.push_back(r.get());
+        if (piece_lines > 0 && sh.lines > piece_lines) {
+            // served in pieces, each a job of a later frame than the one before,
+            // so the first pieces' IQ is copied while the next ones compute
+            int line0 = 0;
+            for (int pg : pieces_of(gi)) {
+                const Shape& ps = shape_of(p, pg);
+                if (ps.samples != sh.samples) fail(BCSIM_ERR_INTERNAL, "piece and request differ in samples per line");
+                auto q = std::make_unique<Request>();
+                q->phantom = p;
+                q->geometry = pg;
+                q->t = t;
+                q->lines = ps.lines;
+                q->samples = ps.samples;
+                q->parent = r.get();
+                q->line0 = line0;
+                q->t_submit = r->t_submit;
+                line0 += ps.lines;
+                r->pieces.push_back(std::move(q));
+            }
+            r->pieces_left = (int)r->pieces.size();
+            for (auto& q : r->pieces) pending.push_back(q.get());
+        } else {
+            pending.push_back(r.get());
+        }
         const uint64_t id = r->id;
         requests[id] = std::move(r);
         cv_sched.notify_one();
@@ -338,8 +415,8 @@ struct bcsim_server {
 
     // ------------------------------------------------------------ scheduling (mu held)
     Frame* free_frame() {
-        for (Frame& f : frames)
-            if (!f.busy) return &f;
+        for (int k = 0; k < contexts; k++)
+            if (!frames[k].busy) return &frames[k];
         return nullptr;
     }
 
@@ -402,7 +479,9 @@ struct bcsim_server {
         (void)hipHostFree(p);
     }
 
-    void finish(Request* r, int status, const std::string& err, double t) {
+    // Returns the request that this completes for its caller: r, its parent when r
+    // is the parent's last piece, or null.
+    Request* finish(Request* r, int status, const std::string& err, double t) {
         r->status = status;
         r->error = err;
         r->done = true;
@@ -412,6 +491,16 @@ struct bcsim_server {
             r->slot->last_used = ++tick;
             r->slot = nullptr;
         }
+        Request* P = r->parent;
+        if (!P) return r;
+        if (status != BCSIM_OK && P->status == BCSIM_OK) {
+            P->status = status;
+            P->error = err;
+        }
+        if (--P->pieces_left > 0) return nullptr;
+        P->done = true;
+        P->t_done = t;
+        return P;
     }
 
     void scheduler_loop() {
@@ -422,9 +511,14 @@ struct bcsim_server {
             if (stop) break;
             Frame* f = free_frame();
             std::vector<Request*> admitted;
+            std::set<const Request*> split;  // requests with a piece in this frame: one piece each
             size_t out_elems = 0;
-            while (!pending.empty() && (int)admitted.size() < max_jobs) {
-                Request* r = pending.front();
+            for (auto it = pending.begin(); it != pending.end() && (int)admitted.size() < max_jobs;) {
+                Request* r = *it;
+                if (r->parent && split.count(r->parent)) {
+                    ++it;  // a later frame takes it
+                    continue;
+                }
                 const Shape& sh = shapes[std::make_pair(r->phantom, r->geometry)];
                 Slot* s = take_idle(r->phantom, r->geometry);
                 const size_t slot_need = s ? 0 : sh.slot_bytes;
@@ -433,7 +527,7 @@ struct bcsim_server {
                 if (!make_room(slot_need + out_need, s)) {
                     if (admitted.empty() && inflight.empty()) {
                         // nothing will free memory: this request can never run
-                        pending.pop_front();
+                        it = pending.erase(it);
                         finish(r, BCSIM_ERR_OUT_OF_MEMORY, "request exceeds the server's memory budget", now_ms());
                         cv_done.notify_all();
                         continue;
@@ -445,7 +539,7 @@ struct bcsim_server {
                     try {
                         slot->sim = make_sim(r->phantom, r->geometry);
                     } catch (const BcsimError& e) {
-                        pending.pop_front();
+                        it = pending.erase(it);
                         finish(r, e.code, e.what(), now_ms());
                         cv_done.notify_all();
                         continue;
@@ -459,8 +553,9 @@ struct bcsim_server {
                 }
                 s->busy = true;
                 r->slot = s;
-                pending.pop_front();
+                it = pending.erase(it);
                 admitted.push_back(r);
+                if (r->parent) split.insert(r->parent);
                 out_elems += (size_t)sh.lines * sh.samples;
             }
             if (admitted.empty()) {
@@ -473,6 +568,7 @@ struct bcsim_server {
             lk.unlock();
             std::string err;
             int code = BCSIM_OK;
+            const double l0 = now_ms();
             try {
                 launch(f, admitted);
             } catch (const BcsimError& e) {
@@ -482,12 +578,18 @@ struct bcsim_server {
                 code = BCSIM_ERR_INTERNAL;
                 err = e.what();
             }
+            const double l1 = now_ms();
             lk.lock();
+            launch_ms.push_back(l1 - l0);
             if (code == BCSIM_OK && f->reqs.empty()) {  // every admitted request failed in preparation
                 f->busy = false;
                 continue;
             }
             if (code != BCSIM_OK) {
+                // copies the launch enqueued before it failed may still write the buffers
+                lk.unlock();
+                (void)hipStreamSynchronize(f->batch.stream);
+                lk.lock();
                 const double t = now_ms();
                 for (Request* r : f->reqs) finish(r, code, err, t);
                 f->reqs.clear();
@@ -531,7 +633,14 @@ struct bcsim_server {
             }
         }
         if (f->reqs.empty()) return;
-        for (Request* r : f->reqs) r->t_launch = now_ms();
+        f->has_piece = false;
+        for (Request* r : f->reqs) {
+            r->t_launch = now_ms();
+            if (Request* P = r->parent) {
+                f->has_piece = true;
+                if (P->t_launch == 0) P->t_launch = r->t_launch;
+            }
+        }
         bcsim_batch& b = f->batch;
         b.use_graph = false;  // the mix changes from frame to frame
         b.autotune = autotune;
@@ -542,18 +651,31 @@ struct bcsim_server {
         b.build();
         const size_t total = b.total_out;
         if (f->d_out.bytes < total * sizeof(float2)) {
+            // by at least half again: a hipFree waits for every stream of the device
             hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
-            f->d_out.ensure(total * sizeof(float2));
+            f->d_out.ensure(std::max(total * sizeof(float2), f->d_out.bytes + f->d_out.bytes / 2));
         }
-        for (Request* r : f->reqs)
-            if (!r->staged && !r->direct)
-                r->staged = pinned.take((size_t)r->lines * r->samples * sizeof(float2), r->staged_cls);
+        for (Request* r : f->reqs) {
+            Request* o = r->parent ? r->parent : r;  // a piece is copied into its parent's buffer
+            if (!o->staged && !o->direct)
+                o->staged = pinned.take((size_t)o->lines * o->samples * sizeof(float2), o->staged_cls);
+        }
         while (f->events.size() < f->reqs.size()) {
             hipEvent_t e = nullptr;
             hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
             f->events.push_back(e);
         }
+        if (!f->computed) hip_check(hipEventCreateWithFlags(&f->computed, hipEventDisableTiming), "hipEventCreate");
+        // A frame with a piece computes after the previous such frame's kernels, so
+        // the pieces of a request finish one after another and each one's copy
+        // overlaps the next one's kernels (rather than all finishing together).
+        if (f->has_piece) {
+            if (last_piece_frame && last_piece_frame != f)
+                hip_check(hipStreamWaitEvent(b.stream, last_piece_frame->computed, 0), "hipStreamWaitEvent");
+            last_piece_frame = f;
+        }
         b.simulate_device(f->d_out.p, total);
+        hip_check(hipEventRecord(f->computed, b.stream), "hipEventRecord");
         // IQ to host memory, smallest first, an event after each request's copy
         std::vector<size_t> idx(f->reqs.size());
         for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
@@ -564,8 +686,10 @@ struct bcsim_server {
         for (size_t k = 0; k < idx.size(); k++) {
             Request* r = f->reqs[idx[k]];
             const float2* src = f->d_out.as<float2>() + b.plan[idx[k]].out_offset;
-            hip_check(hipMemcpyAsync(r->direct ? (void*)r->out : r->staged, src,
-                                     (size_t)r->lines * r->samples * sizeof(float2), hipMemcpyDeviceToHost, b.stream),
+            const Request* o = r->parent ? r->parent : r;
+            float2* dst = static_cast<float2*>(o->direct ? (void*)o->out : o->staged) + (size_t)r->line0 * r->samples;
+            hip_check(hipMemcpyAsync(dst, src, (size_t)r->lines * r->samples * sizeof(float2), hipMemcpyDeviceToHost,
+                                     b.stream),
                       "hipMemcpyAsync");
             hip_check(hipEventRecord(f->events[k], b.stream), "hipEventRecord");
             f->copy_order.push_back(r);
@@ -594,8 +718,8 @@ struct bcsim_server {
                 const double t = now_ms();
                 lk.lock();
                 if (e == hipSuccess) {
-                    finish(r, BCSIM_OK, "", t);
-                    queue_ms.push_back(r->t_launch - r->t_submit);
+                    const Request* whole = finish(r, BCSIM_OK, "", t);
+                    if (whole) queue_ms.push_back(whole->t_launch - whole->t_submit);
                     lines_done += r->lines;
                     t_last_done = t;
                 } else {
@@ -620,6 +744,7 @@ struct bcsim_server {
         else if (key == "copy_ms") out = copy_ms;
         else if (key == "frame_jobs") out = frame_jobs;
         else if (key == "frame_lines") out = frame_lines;
+        else if (key == "launch_ms") out = launch_ms;
         else if (key == "lines_completed") out = {lines_done};
         else if (key == "busy_span_ms") out = {t_first_submit >= 0 && t_last_done >= 0 ? t_last_done - t_first_submit : 0.0};
         else if (key == "budget_bytes") out = {(double)budget};
@@ -666,12 +791,22 @@ BCSIM_API bcsim_status bcsim_server_set_parameter(bcsim_server* s, const char* k
             s->max_jobs = v;
         } else if (k == "autotune") {
             s->autotune = parse_bool(value);
+        } else if (k == "piece_lines") {
+            const int v = parse_int(value);
+            if (v < 0) fail(BCSIM_ERR_INVALID_ARGUMENT, "piece_lines must be >= 0");
+            s->piece_lines = v;
+        } else if (k == "frame_contexts") {
+            const int v = parse_int(value);
+            if (v < 1 || v > bcsim_server::kMaxFrameContexts)
+                fail(BCSIM_ERR_INVALID_ARGUMENT, "frame_contexts must be in [1, 4]");
+            s->contexts = v;
         } else if (k == "reset_stats") {
             s->lat_ms.clear();
             s->queue_ms.clear();
             s->copy_ms.clear();
             s->frame_jobs.clear();
             s->frame_lines.clear();
+            s->launch_ms.clear();
             s->lines_done = 0;
             s->t_first_submit = s->t_last_done = -1;
         } else {
@@ -704,7 +839,7 @@ BCSIM_API bcsim_status bcsim_server_get_output_shape(bcsim_server* s, int phanto
     return guarded([&] {
         if (!s) fail(BCSIM_ERR_INVALID_ARGUMENT, "null server handle");
         std::lock_guard<std::mutex> g(s->mu);
-        const Shape& sh = s->shape_of(phantom_id, geometry_id);
+        const Shape& sh = s->user_shape(phantom_id, geometry_id);
         if (num_lines) *num_lines = sh.lines;
         if (num_samples) *num_samples = sh.samples;
     });

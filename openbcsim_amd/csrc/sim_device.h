@@ -192,6 +192,15 @@ struct FillSeg {
     size_t n;
 };
 
+// The blocks of one batched projection job to order by their work counts:
+// work[n] and the job's n * slices items.
+struct OrderSeg {
+    int* work;
+    int2* items;
+    int n;
+    int slices;
+};
+
 // Launchers (sim_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_fill_zero(float* p, size_t n_floats, hipStream_t stream);
 hipError_t launch_project_fixed(const ProjCommon& c, const FixedSet& s, void* bins, bool complex_bins,
@@ -215,6 +224,8 @@ size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int li
 // Orders blocks 0..n-1 (n <= kMaxOrderedTiles) by work[], heaviest first, into
 // order[] and clears work[] for the next tile launch to fill.
 hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream);
+// launch_order_tiles for several jobs at once: one workgroup per segment.
+hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t stream);
 
 // Batched frames (one launch each for all jobs of a batch): persistent
 // workgroups take items from `counter` (reset by launch_fill_segments).

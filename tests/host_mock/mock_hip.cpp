@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <set>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -90,6 +91,8 @@ void check(const void* p, size_t n, const char* what, const char* op, bool allow
 struct Op {
     std::function<void()> fn;
     std::string name;
+    ihipEvent_t* wait_ev = nullptr;  // hipStreamWaitEvent: runs once that record has completed
+    uint64_t wait_gen = 0;
 };
 
 }  // namespace
@@ -102,7 +105,9 @@ struct ihipStream_t {
     ihipGraph* graph = nullptr;
 };
 struct ihipEvent_t {
-    std::atomic<bool> done{true};
+    // records enqueued / completed; an event is complete when both agree
+    uint64_t recorded = 0;
+    std::atomic<uint64_t> completed{0};
 };
 struct ihipGraph {
     std::vector<Op> ops;
@@ -122,17 +127,23 @@ bool g_exec_started = false;
 std::thread g_exec;
 std::mt19937_64 g_rng(12345);
 
+bool stream_ready(const ihipStream_t* s) {
+    if (s->q.empty() || s->running) return false;
+    const Op& op = s->q.front();
+    return !op.wait_ev || op.wait_ev->completed.load() >= op.wait_gen;
+}
+
 void exec_loop() {
     std::unique_lock<std::mutex> lk(g_q_mu);
     for (;;) {
         g_q_cv.wait(lk, [] {
             for (auto* s : g_streams)
-                if (!s->q.empty() && s->running == 0) return true;
+                if (stream_ready(s)) return true;
             return false;
         });
         std::vector<ihipStream_t*> ready;
         for (auto* s : g_streams)
-            if (!s->q.empty() && s->running == 0) ready.push_back(s);
+            if (stream_ready(s)) ready.push_back(s);
         ihipStream_t* s = ready[g_rng() % ready.size()];
         Op op = std::move(s->q.front());
         s->q.pop_front();
@@ -336,13 +347,36 @@ hipError_t hipEventDestroy(hipEvent_t e) {
     return hipSuccess;
 }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
-    e->done = false;
-    enqueue(s, Op{[e] { e->done = true; }, "eventRecord"});
+    uint64_t gen;
+    {
+        std::lock_guard<std::mutex> g(g_q_mu);
+        gen = ++e->recorded;
+    }
+    enqueue(s, Op{[e, gen] {
+                      uint64_t c = e->completed.load();
+                      while (c < gen && !e->completed.compare_exchange_weak(c, gen)) {
+                      }
+                  },
+                  "eventRecord"});
     return hipSuccess;
 }
 hipError_t hipEventSynchronize(hipEvent_t e) {
     std::unique_lock<std::mutex> lk(g_q_mu);
-    g_done_cv.wait(lk, [&] { return e->done.load(); });
+    const uint64_t gen = e->recorded;
+    g_done_cv.wait(lk, [&] { return e->completed.load() >= gen; });
+    return hipSuccess;
+}
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned int flags) {
+    if (flags != 0) die("hipStreamWaitEvent: flags must be 0");
+    uint64_t gen;
+    {
+        std::lock_guard<std::mutex> g(g_q_mu);
+        gen = e->recorded;
+    }
+    Op op{[] {}, "streamWaitEvent"};
+    op.wait_ev = e;
+    op.wait_gen = gen;
+    enqueue(s, std::move(op));
     return hipSuccess;
 }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
@@ -554,6 +588,32 @@ hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t s) {
     return hipSuccess;
 }
 
+hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t s) {
+    if (n_segs <= 0) return hipSuccess;
+    enqueue(s, Op{[=] {
+                      check(segs, (size_t)n_segs * sizeof(OrderSeg), "segments", "order_segments");
+                      for (int k = 0; k < n_segs; k++) {
+                          const OrderSeg& S = segs[k];
+                          if (S.n <= 0 || S.n > kMaxOrderedTiles) die("order_segments: segment size");
+                          if (S.slices < 1) die("order_segments: slices");
+                          check(S.work, (size_t)S.n * 4, "work", "order_segments");
+                          check(S.items, (size_t)S.n * S.slices * sizeof(int2), "items", "order_segments");
+                          const int job = S.items[0].x;
+                          for (size_t i = 0; i < (size_t)S.n * S.slices; i++)
+                              if (S.items[i].x != job) die("order_segments: items of another job in the segment");
+                          std::vector<int> idx(S.n);
+                          for (int i = 0; i < S.n; i++) idx[i] = i;
+                          std::stable_sort(idx.begin(), idx.end(),
+                                           [&](int a, int b) { return S.work[a] > S.work[b]; });
+                          for (int r = 0; r < S.n; r++)
+                              for (int sl = 0; sl < S.slices; sl++) S.items[(size_t)sl * S.n + r].y = sl * S.n + idx[r];
+                          for (int i = 0; i < S.n; i++) S.work[i] = 0;
+                      }
+                  },
+                  "order_segments"});
+    return hipSuccess;
+}
+
 hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds, int,
                                 hipStream_t s) {
     if (n_items == 0) return hipSuccess;
@@ -574,10 +634,15 @@ hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_it
                           if (tile_lds_bytes(J.g.window_bins, J.flags & kJobPhase, J.g.tile_lines, J.g.list_cap) > lds)
                               die("project_batch: job needs more LDS than launched");
                       }
+                      std::set<std::pair<int, int>> seen;
                       for (int i = 0; i < n_items; i++) {
                           const ProjJob& J = jobs[items[i].x];
                           if (items[i].y < 0 || items[i].y >= J.blocks_per_slice * J.g.slices) die("project_batch: item");
+                          if (!seen.insert({items[i].x, items[i].y}).second) die("project_batch: item listed twice");
                       }
+                      size_t all = 0;
+                      for (int j = 0; j < nj; j++) all += (size_t)jobs[j].blocks_per_slice * jobs[j].g.slices;
+                      if (all != (size_t)n_items) die("project_batch: items do not cover every block");
                       if (lds + 8 > kLds) die("project_batch: LDS");
                   },
                   "project_batch"});

@@ -127,6 +127,46 @@ def test_pinned_outputs_and_completion_order(setup):
     assert rel_l2(a, b) < 1e-6
 
 
+def test_requests_served_in_pieces(setup):
+    """Requests longer than piece_lines run as line ranges in successive frames
+    (at most one piece of a request per frame, kernels of piece frames one after
+    another); each request's IQ still equals its single-simulator frame, into
+    pageable and into pinned outputs, mixed with unsplit requests."""
+    srv, phantoms, geometries, pairs, _ = setup
+    rng = np.random.default_rng(6)
+    big = [pg for pg in pairs if geometries[pg[1]][0].shape[0] in (256, 512)]
+    small = [pg for pg in pairs if geometries[pg[1]][0].shape[0] <= 64]
+    work = [big[i] + (float(rng.uniform(0, 1)),) for i in rng.choice(len(big), 4, replace=False)]
+    work += [small[i] + (float(rng.uniform(0, 1)),) for i in rng.choice(len(small), 4, replace=False)]
+    expected = [single_frame(phantoms, geometries, p, g, t) for p, g, t in work]
+    srv.set_parameter("piece_lines", 100)  # 256 lines: 100 + 100 + 56
+    try:
+        for pinned in (False, True):
+            outs = [srv.alloc_output(p, g) if pinned else None for p, g, _ in work]
+            srv.set_parameter("reset_stats", 1)
+            handles = [srv.submit(p, g, t, out=o) for (p, g, t), o in zip(work, outs)]
+            for k, h in enumerate(handles):
+                iq = srv.wait(h)
+                e = rel_l2(iq, expected[k])
+                print(f"[result] pieces pinned={pinned} request {k} lines={iq.shape[0]}: vs single {e:.2e}, "
+                      f"latency {h.latency_ms:.2f} ms", flush=True)
+                assert iq.shape == expected[k].shape and e < 1e-6
+            lines = srv.stats("frame_lines")
+            assert lines.sum() == sum(x.shape[0] for x in expected)
+            most = max(-(-x.shape[0] // 100) for x in expected)
+            assert len(lines) >= most  # one piece of a request per frame
+            for o in outs:
+                if o is not None:
+                    srv.free_output(o)
+    finally:
+        srv.set_parameter("piece_lines", 1024)
+    from openbcsim_amd.pyrfsim import BcsimError
+    with pytest.raises(BcsimError, match="piece_lines"):
+        srv.set_parameter("piece_lines", -1)
+    with pytest.raises(BcsimError, match="frame_contexts"):
+        srv.set_parameter("frame_contexts", 5)
+
+
 def test_concurrent_submitters(setup):
     srv, phantoms, geometries, pairs, _ = setup
     rng = np.random.default_rng(3)
