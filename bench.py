@@ -209,13 +209,14 @@ def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
             "plan_and_tune_s": round(tune_s, 3)}
 
 
-def run_serving(seconds: float = 10.0, rate: float = 200.0, scale: float = 1.0, seed: int = 11):
+def run_serving(seconds: float = 60.0, rate: float = 200.0, scale: float = 1.0, seed: int = 11,
+                compare_seconds: float = 10.0):
     """Serving loop: 16 resident phantoms (spline M-mode, fixed 2D, 1M-point LUT
     volumes, uneven batched-mix jobs), geometries of 32-512 sector lines and the
-    4096-line pyramid; Poisson arrivals at ``rate`` requests/s for ``seconds``,
-    each naming a phantom, a geometry and a time t, run twice: into pinned
-    output buffers of the server's (the reported p50/p99) and into fresh numpy
-    arrays.  Every (phantom, geometry) pair is requested once before the timed
+    4096-line pyramid; Poisson arrivals at ``rate`` requests/s, each naming a
+    phantom, a geometry and a time t, run twice: for ``seconds`` into pinned
+    output buffers of the server's (the reported p50/p99) and for
+    ``compare_seconds`` into fresh numpy arrays.  Every (phantom, geometry) pair is requested once before the timed
     runs (shape tuning and buffer set-up happen there).  About 1% of the timed
     requests are re-simulated alone on a single simulator and compared (rel L2)."""
     from openbcsim_amd import workloads as W
@@ -240,7 +241,8 @@ def run_serving(seconds: float = 10.0, rate: float = 200.0, scale: float = 1.0, 
         srv.wait(h)
     warm_s = time.perf_counter() - t0
     pinned = poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_outputs=True)
-    pageable = poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_outputs=False)
+    pageable = poisson_run(srv, phantoms, geometries, pairs, rng, rate, min(seconds, compare_seconds),
+                           pinned_outputs=False)
     sat = saturated_serving(srv, sims, geometries, pairs, rng)
     srv.close()
     for s in sims:
@@ -479,7 +481,8 @@ def main():
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scatterer-count scale of the 64 jobs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the per-config (phase off/on) leg")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the timed serving run")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the timed serving run (the numpy-output comparison runs 10 s at most)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--tile", default=None, help="projection launch shape 'windows,lines,block' (default: automatic)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
