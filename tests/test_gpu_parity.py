@@ -400,3 +400,37 @@ def test_host_output_paths_agree():
               flush=True)
         assert np.allclose(host, ref, rtol=1e-6, atol=1e-6 * float(np.abs(ref).max()))
         s.close()
+
+
+def test_phantom_files_through_the_gpu_path(tmp_path):
+    """A fixed phantom, a LUT profile and a scan loaded from HDF5 files of the
+    reference's formats (openbcsim_amd.hdf) into the GPU simulator: the frame of
+    the same arrays in the FP64 oracle, at the parity bar."""
+    from openbcsim_amd import hdf
+    vol = W.volume_lut(20000, n_az=8, n_el=8, seed=9, lut_shape=(64, 16, 16))
+    (r0, r1, l0, l1, e0, e1), table = vol.profile[1], vol.profile[2]
+    o, d, l, t, length = vol.scan
+    paths = {k: str(tmp_path / f"{k}.h5") for k in ("fixed", "lut", "scan")}
+    hdf.save_fixed_scatterers(paths["fixed"], vol.fixed[0])
+    hdf.save_beam_profile(paths["lut"], table, (r0, r1), (l0, l1), (e0, e1))
+    hdf.save_scan_sequence(paths["scan"], o, d, l, t, length)
+    g = gpu_sim()
+    for k, v in vol.params.items():
+        g.set_parameter(k, str(v))
+    g.set_excitation(vol.excitation, vol.center_index, vol.fs, vol.demod_freq)
+    hdf.set_scan_sequence_from_hdf(g, paths["scan"])
+    hdf.set_fixed_scatterers_from_hdf(g, paths["fixed"])
+    hdf.set_beam_profile_from_hdf(g, paths["lut"])
+    iq = g.simulate_frame()
+    g.close()
+    o_ = Oracle()
+    for k, v in vol.params.items():
+        o_.set_parameter(k, str(v))
+    o_.set_excitation(vol.excitation, vol.center_index, vol.fs, vol.demod_freq)
+    o_.set_scan_sequence(o, d, length, l, t)
+    o_.add_fixed_scatterers(vol.fixed[0])
+    o_.set_lut_beam_profile(r0, r1, l0, l1, e0, e1, table / table.max())
+    ref = o_.simulate_frame()
+    e = rel_l2(iq, ref)
+    print(f"[result] phantom files through the GPU path: rel L2 vs oracle {e:.2e}", flush=True)
+    assert np.abs(ref).max() > 0 and e <= 1e-4
