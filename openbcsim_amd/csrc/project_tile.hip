@@ -145,8 +145,11 @@ __device__ __forceinline__ void emit(const ProjCommon& c, float* lbins, int b0, 
 
 // One (tile, window, slice) block of work: everything a workgroup of the tile
 // kernel does.  `wg` = tile * windows + window; `lds` is the dynamic LDS and
-// `list_n` a workgroup-shared counter.
-template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
+// `list_n` a workgroup-shared counter.  CP_AHEAD: a spline's active control
+// points are all loaded before the first is used (the tile kernel; the
+// persistent batch kernel, which holds every variant, keeps one control point
+// at a time: measured faster there).
+template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED, bool CP_AHEAD = true>
 __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams& cp, const FixedSet& fs,
                                            const SplineSet& ss, const ChunkBox* __restrict__ boxes, int nchunks,
                                            float* __restrict__ out, const TileLaunch& g, int wg, int slice,
@@ -271,15 +274,40 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     const SplineLineBasis& Bk = tb[k];
                     // CpuAlgorithm.cpp:137-140: float sum from zero, in control-point order
                     float px = 0.0f, py = 0.0f, pz = 0.0f;
-                    for (int i = 0; i < Bk.count; i++) {
-                        const size_t idx = (size_t)(Bk.lo + i) * pitch + s;
-                        px = px + ss.cx[idx] * Bk.b[i];
-                        py = py + ss.cy[idx] * Bk.b[i];
-                        pz = pz + ss.cz[idx] * Bk.b[i];
+                    if (CP_AHEAD) {
+                        // every active control point's loads before the first is used: one
+                        // memory latency per hit, not one per control point
+                        const int count = __builtin_amdgcn_readfirstlane(Bk.count);  // one line: wave-uniform
+                        const size_t base = (size_t)__builtin_amdgcn_readfirstlane(Bk.lo) * pitch + s;
+                        float cx[kMaxSplineDegree + 1], cy[kMaxSplineDegree + 1], cz[kMaxSplineDegree + 1];
+#pragma unroll
+                        for (int i = 0; i <= kMaxSplineDegree; i++) {
+                            if (i < count) {
+                                const size_t idx = base + (size_t)i * pitch;
+                                cx[i] = ss.cx[idx];
+                                cy[i] = ss.cy[idx];
+                                cz[i] = ss.cz[idx];
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i <= kMaxSplineDegree; i++) {
+                            if (i < count) {
+                                px = px + cx[i] * Bk.b[i];
+                                py = py + cy[i] * Bk.b[i];
+                                pz = pz + cz[i] * Bk.b[i];
+                            }
+                        }
+                    } else {
+                        for (int i = 0; i < Bk.count; i++) {
+                            const size_t idx = (size_t)(Bk.lo + i) * pitch + s;
+                            px = px + ss.cx[idx] * Bk.b[i];
+                            py = py + ss.cy[idx] * Bk.b[i];
+                            pz = pz + ss.cz[idx] * Bk.b[i];
+                        }
                     }
                     const Local q = local_coords<ARC>(tl[k], px, py, pz);
                     double ti;
-                    const int bin = radial_bin(c, q.r, ti);
+                    const int bin = radial_bin_for<PHASE>(c, q.r, ti);
                     emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, amp);
                 }
             } else if (SHARED && ARC) {
@@ -287,7 +315,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 const float dx = p.x - org_x, dy = p.y - org_y, dz = p.z - org_z;
                 const float absd = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
                 double ti_pos;
-                const int bin_pos = radial_bin(c, absd, ti_pos);
+                const int bin_pos = radial_bin_for<PHASE>(c, absd, ti_pos);
                 const int lb_pos = bin_pos - b0;
                 const bool in_pos = bin_pos >= 0 && lb_pos >= 0 && lb_pos < nb;
                 LutRadial R{};
@@ -306,7 +334,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                         add_bin_rot<PHASE>(bins + k * stride, lb_pos, w, co_pos, s_pos);
                     } else {
                         double ti;
-                        const int bin = radial_bin(c, -absd, ti);
+                        const int bin = radial_bin_for<PHASE>(c, -absd, ti);
                         emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w);
                     }
                 };
@@ -334,7 +362,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     mask &= mask - 1;
                     const Local q = local_coords<ARC>(tl[k], p.x, p.y, p.z);
                     double ti;
-                    const int bin = radial_bin(c, q.r, ti);
+                    const int bin = radial_bin_for<PHASE>(c, q.r, ti);
                     emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, p.w);
                 }
             }
@@ -461,8 +489,8 @@ hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t s
 template <bool ARC, bool PHASE, bool LUT>
 __device__ __forceinline__ void tile_block_job(const ProjJob& J, int wg, int slice, float* lds, int& list_n) {
     if (J.flags & kJobSpline)
-        tile_block<ARC, PHASE, LUT, true, false>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg, slice,
-                                                 lds, list_n);
+        tile_block<ARC, PHASE, LUT, true, false, false>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg,
+                                                        slice, lds, list_n);
     else if (ARC && (J.flags & kJobShared))
         tile_block<ARC, PHASE, LUT, false, true>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg, slice,
                                                  lds, list_n);
