@@ -118,42 +118,6 @@ __device__ __forceinline__ int radial_bin(const ProjCommon& c, float r, double& 
     return (int)fl;
 }
 
-// radial_bin where the sub-sample position is not needed (no phase delay), in
-// float arithmetic: r * scale / denom + 0.5 is formed as an unevaluated sum
-// t_hi + (err + s), where r * bin_factor = p + e exactly (fused multiply-add),
-// s adds r * bin_factor_lo (bin_factor + bin_factor_lo = scale / denom to 2^-48),
-// and t_hi + err = p + 0.5 exactly (two-sum).  The fraction f of that sum above
-// floor(t_hi) is within 2.5e-7 of the exact one (the last rounding, of a number
-// below 2, dominates), so wherever it lies farther than 2.5e-7 from 0 and 1 the
-// bin is floor(t_hi) + floor(f); closer to an edge (and for non-finite r),
-// radial_bin's double evaluation decides.
-__device__ __forceinline__ int radial_bin_nophase(const ProjCommon& c, float r) {
-    const float p = r * c.bin_factor;
-    const float e = fmaf(r, c.bin_factor, -p);
-    const float s = e + r * c.bin_factor_lo;
-    const float t_hi = p + 0.5f;
-    const float bb = t_hi - p;
-    const float err = (p - (t_hi - bb)) + (0.5f - bb);
-    const float fl = floorf(t_hi);
-    const float f = (t_hi - fl) + (err + s);
-    const float k = floorf(f);
-    const float fr = f - k;
-    if (fr > 2.5e-7f && fr < 1.0f - 2.5e-7f) {
-        const float bin = fl + k;
-        return (bin >= 0.0f && bin < (float)c.num_rf) ? (int)bin : -1;
-    }
-    double ti;
-    return radial_bin(c, r, ti);
-}
-
-// The bin of r, and for PHASE also its sub-sample position true_index.
-template <bool PHASE>
-__device__ __forceinline__ int radial_bin_for(const ProjCommon& c, float r, double& true_index) {
-    if (PHASE) return radial_bin(c, r, true_index);
-    true_index = 0.0;
-    return radial_bin_nophase(c, r);
-}
-
 // Add one scatterer's contribution to its bin in LDS (CpuAlgorithm.cpp:77-95).
 template <bool PHASE, bool LUT>
 __device__ __forceinline__ void accumulate(const ProjCommon& c, float r, float l, float e, int bin,

@@ -307,7 +307,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     }
                     const Local q = local_coords<ARC>(tl[k], px, py, pz);
                     double ti;
-                    const int bin = radial_bin_for<PHASE>(c, q.r, ti);
+                    const int bin = radial_bin(c, q.r, ti);
                     emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, amp);
                 }
             } else if (SHARED && ARC) {
@@ -315,7 +315,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 const float dx = p.x - org_x, dy = p.y - org_y, dz = p.z - org_z;
                 const float absd = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
                 double ti_pos;
-                const int bin_pos = radial_bin_for<PHASE>(c, absd, ti_pos);
+                const int bin_pos = radial_bin(c, absd, ti_pos);
                 const int lb_pos = bin_pos - b0;
                 const bool in_pos = bin_pos >= 0 && lb_pos >= 0 && lb_pos < nb;
                 LutRadial R{};
@@ -334,7 +334,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                         add_bin_rot<PHASE>(bins + k * stride, lb_pos, w, co_pos, s_pos);
                     } else {
                         double ti;
-                        const int bin = radial_bin_for<PHASE>(c, -absd, ti);
+                        const int bin = radial_bin(c, -absd, ti);
                         emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w);
                     }
                 };
@@ -362,7 +362,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     mask &= mask - 1;
                     const Local q = local_coords<ARC>(tl[k], p.x, p.y, p.z);
                     double ti;
-                    const int bin = radial_bin_for<PHASE>(c, q.r, ti);
+                    const int bin = radial_bin(c, q.r, ti);
                     emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, p.w);
                 }
             }

@@ -64,8 +64,6 @@ struct ProjCommon {
     double bin_scale;
     double bin_denom;
     double bin_inv_denom;
-    float bin_factor;     // scale / denom as bin_factor + bin_factor_lo (two floats), for
-    float bin_factor_lo;  // the float evaluation of the bin away from bin edges
     // sub-sample phase: phase = (bin - true_index) * phase_per_sample, true_index = r*scale/denom
     float phase_per_sample;
     ProfileParams prof;

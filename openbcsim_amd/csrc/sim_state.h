@@ -940,11 +940,6 @@ struct bcsim_sim {
             c.bin_denom = (double)sound_speed;
         }
         c.bin_inv_denom = 1.0 / c.bin_denom;
-        {
-            const double k = c.bin_scale / c.bin_denom;
-            c.bin_factor = (float)k;
-            c.bin_factor_lo = (float)(k - (double)c.bin_factor);
-        }
         c.phase_per_sample = (float)(2.0 * M_PI * (double)f_demod / (double)fs);
         ProfileParams& P = c.prof;
         if (profile == kProfileGaussian) {
