@@ -92,13 +92,18 @@ class _Reader:
                 raise H5Error(f"object header version {self.b[p]} not supported")
             size = self.uint(p + 8, 4)
             self._v1_chunk(p + 16, p + 16 + size, out)
-        # continuation blocks (message 0x10) append their messages
+        # continuation blocks (message 0x10) append their messages; a block met
+        # twice (a damaged file) would repeat forever
         k = 0
+        blocks = set()
         while k < len(out):
             t, d = out[k]
             if t == 0x10:
                 ca = int.from_bytes(d[:self.so], "little")
                 cl = int.from_bytes(d[self.so:self.so + self.sl], "little")
+                if ca in blocks or not 0 <= self.at(ca) < len(self.b):
+                    raise H5Error("damaged object header continuation")
+                blocks.add(ca)
                 c = self.at(ca)
                 if self.b[c:c + 4] == b"OCHK":
                     self._v2_chunk(c + 4, c + cl - 4, self._v2_flags(a), out)
