@@ -434,3 +434,34 @@ def test_phantom_files_through_the_gpu_path(tmp_path):
     e = rel_l2(iq, ref)
     print(f"[result] phantom files through the GPU path: rel L2 vs oracle {e:.2e}", flush=True)
     assert np.abs(ref).max() > 0 and e <= 1e-4
+
+
+@pytest.mark.parametrize("phase", [False, True])
+def test_lut_unequal_lateral_and_elevation(phase):
+    """A LUT of unequal lateral and elevational sizes and extents with random
+    samples (no symmetry between l and e): a swapped lateral/elevational index
+    or extent, or a transposed table, moves the frame far from the oracle's."""
+    import dataclasses
+    rng = np.random.default_rng(21)
+    vol = W.volume_lut(20000, n_az=6, n_el=5, seed=22, line_length=0.06, lut_shape=(40, 12, 20),
+                       phase_delay=phase)
+    table = rng.uniform(0.0, 1.0, (40, 12, 20)).astype(np.float32)
+    wl = dataclasses.replace(vol, profile=("lut", (0.0, 0.06, -0.012, 0.009, -0.006, 0.014), table))
+    o_ = Oracle()
+    wl.apply(o_)
+    ref = o_.simulate_frame()
+    for mode in ("tiled", "direct"):
+        g = gpu_sim()
+        g.set_parameter("projection_mode", mode)
+        wl.apply(g)
+        iq = g.simulate_frame()
+        g.close()
+        e = rel_l2(iq, ref)
+        print(f"[result] unequal LUT phase={phase} {mode}: rel L2 vs oracle {e:.2e}", flush=True)
+        assert np.abs(ref).max() > 0 and e <= 1e-4
+    # the same table transposed in l and e (extents swapped) is a different frame
+    wt = dataclasses.replace(vol, profile=("lut", (0.0, 0.06, -0.006, 0.014, -0.012, 0.009),
+                                            np.ascontiguousarray(table.transpose(0, 2, 1))))
+    o_t = Oracle()
+    wt.apply(o_t)
+    assert rel_l2(o_t.simulate_frame(), ref) > 1e-2
