@@ -465,3 +465,27 @@ def test_lut_unequal_lateral_and_elevation(phase):
     o_t = Oracle()
     wt.apply(o_t)
     assert rel_l2(o_t.simulate_frame(), ref) > 1e-2
+
+
+@pytest.mark.parametrize("arc", [True, False])
+def test_linear_scan_with_lut(arc):
+    """Lines that do not share an origin (a linear scan: the per-line cull and
+    projection of the tile kernel) with a LUT profile, arc projection on and off."""
+    import dataclasses
+    rng = np.random.default_rng(23)
+    vol = W.volume_lut(20000, n_az=2, n_el=2, seed=24, line_length=0.06, lut_shape=(40, 10, 14))
+    o, d, l, t, length = W.linear_scan(24, 0.04, 0.06)
+    table = rng.uniform(0.0, 1.0, (40, 10, 14)).astype(np.float32)
+    params = dict(vol.params, use_arc_projection="on" if arc else "off")
+    wl = dataclasses.replace(vol, scan=(o, d, l, t, length), params=params,
+                             profile=("lut", (0.0, 0.06, -0.01, 0.01, -0.007, 0.007), table))
+    o_ = Oracle()
+    wl.apply(o_)
+    ref = o_.simulate_frame()
+    g = gpu_sim()
+    wl.apply(g)
+    iq = g.simulate_frame()
+    g.close()
+    e = rel_l2(iq, ref)
+    print(f"[result] linear scan with LUT arc={arc}: rel L2 vs oracle {e:.2e}", flush=True)
+    assert np.abs(ref).max() > 0 and e <= 1e-4
