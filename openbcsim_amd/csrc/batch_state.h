@@ -219,7 +219,7 @@ struct bcsim_batch {
                 if (tile_lines > 0) {
                     TileTune tt = s->tune;
                     tt.lines = tile_lines;
-                    J.g = s->tile_launch(t.nchunks, tt);
+                    J.g = s->tile_launch(t.nchunks, tt, is_spline);
                 }
                 J.g.block = 1024;
                 J.g.chunks = t.chunk_order;
