@@ -28,7 +28,7 @@ def test_header_declares_the_reference_interface():
               "bcsim_set_scan_sequence", "bcsim_set_excitation", "bcsim_set_analytical_profile",
               "bcsim_set_lut_profile", "bcsim_simulate_lines", "bcsim_get_debug_data",
               "bcsim_get_total_num_scatterers", "bcsim_set_logger",
-              # north-star names
+              # set_* and create_simulator aliases of the same entry points
               "bcsim_create_simulator", "bcsim_set_fixed_scatterers", "bcsim_set_spline_scatterers"]:
         assert n in names, n
 

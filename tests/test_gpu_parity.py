@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C-ABI) with the FP64 oracle and with the
 reference's own CPU algorithm.
 
-Tolerance (the bar of the north star): relative L2 error of the complex IQ
+Tolerance: relative L2 error of the complex IQ
 frame <= 1e-4 against the FP64 oracle, because float atomics make the summation
 order nondeterministic.  The projection bins are checked on their own too:
 every bin that carries signal is the bin the oracle chose (indices are integer
