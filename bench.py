@@ -50,13 +50,12 @@ MEASURED_TRAFFIC = {"volume_lut": {"bytes": 1.867e6 * 1024 * 2.0 + 1.573e5 * 102
 
 def build_workload(name: str):
     from openbcsim_amd import workloads as W
-    if name == "volume_lut":
-        return W.volume_lut()
-    if name == "spline_mmode":
-        return W.spline_mmode()
-    if name == "fixed_sector_2d":
-        return W.fixed_sector_2d()
-    raise SystemExit(f"unknown workload {name}")
+    makers = {"volume_lut": W.volume_lut, "spline_mmode": W.spline_mmode, "fixed_sector_2d": W.fixed_sector_2d}
+    base = name[:-len("_phase")] if name.endswith("_phase") else name
+    if base not in makers:
+        raise SystemExit(f"unknown workload {name}")
+    # "<workload>_phase": the same workload with sub-sample phase delay on
+    return makers[base](phase_delay=True) if base != name else makers[base]()
 
 
 def projection_key(wl) -> str:
@@ -514,7 +513,7 @@ def main():
     if proj_ms:
         algo_bytes = BYTES_PER_EVAL[kind] * evals
         achieved = algo_bytes / (proj_ms * 1e-3) / 1e9
-        measured = MEASURED_TRAFFIC.get(wl.name) if not args.tile else None
+        measured = MEASURED_TRAFFIC.get(args.workload) if not args.tile else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": round(measured["bytes"]) if measured else None,
@@ -536,9 +535,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded phantom, scan and LUT)",
-        "config": {"workload": wl.name, "lines": L, "scatterers": N, "samples_per_line": S,
+        "config": {"workload": args.workload, "lines": L, "scatterers": N, "samples_per_line": S,
                    "fft_len": wl.fft_len(), "decimation": int(wl.params.get("radial_decimation", 1)),
-                   "profile": wl.profile[0]},
+                   "profile": wl.profile[0], "phase_delay": wl.params.get("phase_delay", "off")},
         "scatterer_evals_per_s": round(evals / (ms * 1e-3), 1),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "pcie_inclusive_lines_per_s": round(L / (host_ms * 1e-3), 1),
