@@ -9,7 +9,8 @@ bracketed by device synchronisation.  Prints one JSON line.
 
 Workload (default): the 3D volume scan quoted "at 1M scatterers" -- 1M fixed
 float32 points in a 6x6x11 cm box, 64x64 = 4096 pyramidal lines over +-30 deg,
-a 256x64x64 float32 LUT beam profile, FFT length 16384, decimation 2.
+a 256x64x64 float32 LUT beam profile, FFT length 16384, decimation 2.  Others:
+spline_mmode, fixed_sector_2d, and each name with "_phase" appended (phase delay on).
 
 roofline: the projection kernel's average duration is measured with HIP events
 on the simulator's stream (store_kernel_details) over the same K frames; its
