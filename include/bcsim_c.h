@@ -23,6 +23,7 @@
 #define BCSIM_C_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {

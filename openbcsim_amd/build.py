@@ -15,6 +15,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -40,16 +41,15 @@ def _run(cmd):
 
 def build(verbose: bool = False) -> str:
     """Compile every HIP source and link the shared library; returns its path."""
-    objs = []
     obj_dir = os.path.join(PKG, "_build")
     os.makedirs(obj_dir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
-        objs.append(obj)
+    objs = [os.path.join(obj_dir, os.path.splitext(src)[0] + ".o") for src in SOURCES]
+    cmds = [[HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] for src, obj in zip(SOURCES, objs)]
+    # one compiler process per source (each is single-threaded)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, os.cpu_count() or 1))) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
@@ -65,13 +65,20 @@ def build_oracle() -> None:
         _run(["make", "-s", "-C", oracle, "-B", "ref"])
 
 
+C_CALLER_FLAGS = ["-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic"]
+
+
 def build_cxx_check() -> str:
     """Compile tests/cxx/facade_check.cpp (a C++ IAlgorithm user of the facade
-    header) against the built library, for the GPU tests to run."""
+    header) and tests/cxx/c_caller.c (a C99 user of the C-ABI, warnings as errors)
+    against the built library, for the GPU tests to run."""
     out = os.path.join(PKG, "_build", "facade_check")
     _run(["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "include"),
           os.path.join(ROOT, "tests", "cxx", "facade_check.cpp"), "-o", out, "-L", PKG, "-lbcsim_hip",
           "-Wl,-rpath,$ORIGIN/.."])
+    _run(["gcc", *C_CALLER_FLAGS, "-O2", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cxx", "c_caller.c"), "-o", os.path.join(PKG, "_build", "c_caller"),
+          "-L", PKG, "-lbcsim_hip", "-lm", "-Wl,-rpath,$ORIGIN/.."])
     return out
 
 

@@ -46,6 +46,11 @@ struct bcsim_batch {
     std::vector<JobPlan> plan;
     size_t total_out = 0;
     DevBuf d_proj_jobs, d_proj_items, d_conv_jobs, d_conv_items, d_fill, d_counters, d_order_segs;
+    // Per job, the noise frame counter of the frame in flight: set to the
+    // simulator's counter minus one by build(), advanced by the fill kernel at the
+    // start of every frame (graph replays included), so batched frames draw the
+    // same noise as the simulator's own consecutive frames would.
+    DevBuf d_noise;
     int n_proj_items = 0, n_conv_items = 0, n_fill = 0, n_order_segs = 0;
     size_t proj_lds = 0;
     struct Render {
@@ -193,6 +198,7 @@ struct bcsim_batch {
         std::vector<FillSeg> fills;
         std::vector<std::pair<int, int*>> ordered;  // (projection job, its work counts)
         std::vector<int> job_first_target, job_targets;
+        std::vector<uint32_t> noise_init;
         size_t offset = 0;
         proj_lds = 0;
         for (bcsim_sim* s : sims) {
@@ -248,7 +254,10 @@ struct bcsim_batch {
             }
             job_first_target.push_back((int)pjobs.size() - (int)targets.size());
             job_targets.push_back((int)targets.size());
+            const uint32_t ctr = s->noise_counter;
             cjobs.push_back(s->conv_params(nullptr));  // iq pointer set per call (offset into the output)
+            s->noise_counter = ctr;  // a batched frame advances it when it is enqueued
+            noise_init.push_back(ctr - 1u);
             cjobs.back().radix = s->plan_radix(tune_now);
             s->quiesce();  // renders and trials on the job's own stream are done before the batch runs
         }
@@ -294,6 +303,8 @@ struct bcsim_batch {
         grow(d_conv_items, citems.size() * sizeof(int2));
         grow(d_conv_jobs, cjobs.size() * sizeof(ConvParams));
         grow(d_counters, 2 * sizeof(int));
+        grow(d_noise, cjobs.size() * sizeof(uint32_t));
+        for (size_t k = 0; k < cjobs.size(); k++) cjobs[k].noise_counter_ptr = d_noise.as<uint32_t>() + k;
         std::vector<OrderSeg> order_segs;
         for (const auto& o : ordered) {
             const ProjJob& J = pjobs[o.first];
@@ -306,6 +317,7 @@ struct bcsim_batch {
         st.add(d_fill, fills.data(), fills.size() * sizeof(FillSeg));
         st.add(d_order_segs, order_segs.data(), order_segs.size() * sizeof(OrderSeg));
         st.add(d_conv_items, citems.data(), citems.size() * sizeof(int2));
+        st.add(d_noise, noise_init.data(), noise_init.size() * sizeof(uint32_t));
         st.upload();
         conv_host = cjobs;
         n_proj_items = (int)pitems.size();
@@ -340,7 +352,9 @@ struct bcsim_batch {
     }
 
     void enqueue() {
-        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), 2, stream), "fill kernel");
+        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), 2, d_noise.as<uint32_t>(),
+                                       (int)plan.size(), stream),
+                  "fill kernel");
         for (const Render& r : renders)
             hip_check(launch_render_spline(r.set, r.basis, r.out, stream), "render kernel");
         hip_check(launch_order_segments(d_order_segs.as<OrderSeg>(), n_order_segs, stream), "block order kernel");
@@ -383,6 +397,7 @@ struct bcsim_batch {
             set_output(static_cast<float2*>(d_out));
             enqueue();
         }
+        for (bcsim_sim* s : sims) s->noise_counter++;  // this frame's counter is in flight on the device
         if (store_details) {
             hip_check(hipEventRecord(b, stream), "hipEventRecord");
             hip_check(hipEventSynchronize(b), "hipEventSynchronize");

@@ -503,8 +503,10 @@ struct bcsim_server {
         return P;
     }
 
+    // The worker threads start with the server, before any phantom names the
+    // device: launch() selects it for every frame, and the completer once it has a
+    // frame (both read `device` after mu has ordered them behind add_phantom).
     void scheduler_loop() {
-        (void)hipSetDevice(device < 0 ? 0 : device);
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             cv_sched.wait(lk, [&] { return stop || (!pending.empty() && free_frame()); });
@@ -704,12 +706,16 @@ struct bcsim_server {
     }
 
     void completer_loop() {
-        (void)hipSetDevice(device < 0 ? 0 : device);
         std::unique_lock<std::mutex> lk(mu);
+        int current_device = -1;
         for (;;) {
             cv_comp.wait(lk, [&] { return !inflight.empty() || sched_exited; });
             if (inflight.empty()) break;
             Frame* f = inflight.front();
+            if (current_device != device) {  // mu held: set by add_phantom before any frame
+                (void)hipSetDevice(device);  // a failure shows in the event waits below
+                current_device = device;
+            }
             // requests complete one by one, as their copies land
             for (size_t k = 0; k < f->copy_order.size(); k++) {
                 Request* r = f->copy_order[k];

@@ -116,6 +116,9 @@ struct ConvParams {
     float noise_amplitude;
     uint32_t noise_seed;
     uint32_t noise_counter;
+    // when set, the frame counter is read from device memory instead (batched
+    // frames: launch_fill_segments advances it at the start of every frame)
+    const uint32_t* noise_counter_ptr;
     int radix;              // largest FFT pass radix: 4 or 8
 };
 
@@ -228,13 +231,14 @@ hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream);
 hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t stream);
 
 // Batched frames (one launch each for all jobs of a batch): persistent
-// workgroups take items from `counter` (reset by launch_fill_segments).
+// workgroups take items from `counter` (reset by launch_fill_segments, which
+// also adds one to each of the n_noise per-job noise frame counters).
 hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds,
                                 int workgroups, hipStream_t stream);
 hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int n_items, int* counter,
                                  int workgroups, hipStream_t stream);
 hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, int n_counters,
-                                hipStream_t stream);
+                                uint32_t* noise_counters, int n_noise, hipStream_t stream);
 size_t convolve_batch_lds_bytes();
 
 }  // namespace bcsim_hip

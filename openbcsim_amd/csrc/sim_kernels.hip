@@ -450,7 +450,8 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
         // complex bins (phase delay) or noise: the full N-point transform
         const int nrf = p.num_rf;
         const float amp = p.noise_amplitude;
-        const uint32_t seed = p.noise_seed, counter = p.noise_counter;
+        const uint32_t seed = p.noise_seed;
+        const uint32_t counter = p.noise_counter_ptr ? *p.noise_counter_ptr : p.noise_counter;
         const auto load = [=](int n) -> float2 {
             if (n >= nrf) return make_float2(0.f, 0.f);
             float2 v;
@@ -522,9 +523,14 @@ hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int 
     return hipGetLastError();
 }
 
-// Zero many regions and reset work counters in one launch (graph-safe: no memset nodes).
-__global__ void fill_segments_kernel(const FillSeg* segs, int n_segs, int* counters, int n_counters) {
+// Zero many regions, reset work counters and advance the jobs' noise frame
+// counters in one launch (graph-safe: no memset nodes, and a replayed graph
+// advances the noise stream like a fresh launch).
+__global__ void fill_segments_kernel(const FillSeg* segs, int n_segs, int* counters, int n_counters,
+                                     uint32_t* noise_counters, int n_noise) {
     if (blockIdx.x == 0 && (int)threadIdx.x < n_counters) counters[threadIdx.x] = 0;
+    if (blockIdx.x == 1)
+        for (int i = threadIdx.x; i < n_noise; i += blockDim.x) noise_counters[i] += 1u;
     for (int s = 0; s < n_segs; s++) {
         float* p = segs[s].p;
         const size_t n = segs[s].n;
@@ -534,8 +540,9 @@ __global__ void fill_segments_kernel(const FillSeg* segs, int n_segs, int* count
 }
 
 hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, int n_counters,
-                                hipStream_t stream) {
-    hipLaunchKernelGGL(fill_segments_kernel, dim3(1024), dim3(256), 0, stream, segs, n_segs, counters, n_counters);
+                                uint32_t* noise_counters, int n_noise, hipStream_t stream) {
+    hipLaunchKernelGGL(fill_segments_kernel, dim3(1024), dim3(256), 0, stream, segs, n_segs, counters, n_counters,
+                       noise_counters, n_noise);
     return hipGetLastError();
 }
 

@@ -263,6 +263,16 @@ inline int parse_int(const std::string& v) {
     }
 }
 
+// A whole decimal number in [0, 2^32); anything else (signs, trailing text,
+// overflow) is an invalid argument.
+inline uint32_t parse_u32(const std::string& v) {
+    if (v.empty() || v.size() > 10 || v.find_first_not_of("0123456789") != std::string::npos)
+        fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid unsigned 32-bit integer: '" + v + "'");
+    const unsigned long long x = std::stoull(v);
+    if (x > 0xffffffffull) fail(BCSIM_ERR_INVALID_ARGUMENT, "invalid unsigned 32-bit integer: '" + v + "'");
+    return (uint32_t)x;
+}
+
 inline float parse_float(const std::string& v) {
     try {
         return std::stof(v);
@@ -383,7 +393,7 @@ struct bcsim_sim {
         } else if (key == "noise_amplitude") {
             noise_amplitude = parse_float(value);
         } else if (key == "noise_seed") {
-            noise_seed = (uint32_t)std::stoul(value);
+            noise_seed = parse_u32(value);
             noise_counter = 0;
         } else if (key == "use_arc_projection") {
             arc = parse_bool(value);
@@ -982,8 +992,9 @@ struct bcsim_sim {
     struct Timed {
         bcsim_sim* s;
         const char* key;
+        const char* alias;  // a second key that records the same duration, or null
         hipEvent_t a = nullptr, b = nullptr;
-        Timed(bcsim_sim* s_, const char* k) : s(s_), key(k) {
+        Timed(bcsim_sim* s_, const char* k, const char* al = nullptr) : s(s_), key(k), alias(al) {
             if (!s->store_details) return;
             hip_check(hipEventCreate(&a), "hipEventCreate");
             hip_check(hipEventCreate(&b), "hipEventCreate");
@@ -992,7 +1003,7 @@ struct bcsim_sim {
         void stop() {
             if (!a) return;
             hip_check(hipEventRecord(b, s->stream), "hipEventRecord");
-            s->pending.push_back({key, a, b});
+            s->pending.push_back({key, alias ? alias : "", a, b});
             a = b = nullptr;
         }
         ~Timed() {
@@ -1001,7 +1012,7 @@ struct bcsim_sim {
         }
     };
     struct Pending {
-        std::string key;
+        std::string key, alias;
         hipEvent_t a, b;
     };
     std::vector<Pending> pending;
@@ -1009,8 +1020,10 @@ struct bcsim_sim {
     void collect_timings() {
         for (auto& p : pending) {
             float ms = 0.f;
-            if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+            if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
                 debug[p.key].push_back((double)ms);
+                if (!p.alias.empty()) debug[p.alias].push_back((double)ms);
+            }
             (void)hipEventDestroy(p.a);
             (void)hipEventDestroy(p.b);
         }
@@ -1319,9 +1332,24 @@ struct bcsim_sim {
         return r;
     }
 
+    // Debug data keys the reference records on every frame with
+    // store_kernel_details (GpuAlgorithm.cpp:193-358) that have no kernel of their
+    // own here: one stream per frame (stream number 0 for every line), and the
+    // multiply, inverse transform and demodulation fused into the convolution
+    // kernel, whose duration is kernel_forward_fft_ms (and
+    // kernel_convolve_demodulate_ms).
+    void record_fused_keys() {
+        debug["stream_numbers"].assign(lines.size(), 0.0);
+        for (const char* k : {"kernel_multiply_fft_ms", "kernel_inverse_fft_ms", "kernel_demodulate_ms"})
+            debug[k].push_back(0.0);
+    }
+
     void enqueue_frame(float2* d_out) {
         prepare();
-        if (store_details) debug.clear();
+        if (store_details) {
+            debug.clear();
+            record_fused_keys();
+        }
         const bool cplx = phase_delay;
         const size_t L = lines.size();
         const size_t bin_floats = L * (size_t)num_rf * (cplx ? 2 : 1);
@@ -1355,6 +1383,8 @@ struct bcsim_sim {
             Timed t(this, "kernel_memset_ms");
             hip_check(launch_fill_zero(bins, bin_floats, stream), "fill kernel");
             t.stop();
+        } else if (store_details) {
+            debug["kernel_memset_ms"].push_back(0.0);  // every bin is stored by the projection
         }
         for (size_t i = 0; i < targets.size(); i++) {
             const ProjTarget& t = targets[i];
@@ -1387,7 +1417,7 @@ struct bcsim_sim {
         ConvParams p = conv_params(d_out);
         p.radix = frame_radix;
         {
-            Timed t(this, "kernel_convolve_demodulate_ms");
+            Timed t(this, "kernel_convolve_demodulate_ms", "kernel_forward_fft_ms");
             hip_check(launch_convolve_demod(p, stream), "convolve/demodulate kernel");
             t.stop();
         }
@@ -1420,6 +1450,7 @@ struct bcsim_sim {
         p.noise_amplitude = noise_amplitude > 0.0f ? noise_amplitude : 0.0f;
         p.noise_seed = noise_seed;
         p.noise_counter = noise_counter++;
+        p.noise_counter_ptr = nullptr;
         p.radix = kDefaultRadix;
         return p;
     }
@@ -1440,7 +1471,10 @@ struct bcsim_sim {
         if (nl) *nl = (int)lines.size();
         if (ns) *ns = n_out;
         enqueue_frame(static_cast<float2*>(d_out));
-        if (store_details) collect_timings();
+        if (store_details) {
+            debug["kernel_memcpy_ms"].push_back(0.0);  // the IQ stays in device memory
+            collect_timings();
+        }
     }
 
     void simulate_host(float* out, size_t capacity, int* nl, int* ns) {

@@ -194,7 +194,11 @@ int main(int argc, char** argv) {
     const int poisson_n = argc > 2 ? atoi(argv[2]) : 2000;
     const double rate = argc > 3 ? atof(argv[3]) : 200.0;
     const bool do_batch = argc > 4 ? atoi(argv[4]) != 0 : true;
-    fprintf(stderr, "[driver] scale=%g poisson=%d rate=%g batch=%d\n", scale, poisson_n, rate, (int)do_batch);
+    // light: fewer jobs, repetitions and requests (the sanitizer builds run 5-20x slower)
+    const bool light = argc > 5 && atoi(argv[5]) != 0;
+    const int n_jobs = light ? 16 : 64, n_sat = light ? 60 : 300, n_bursts = light ? 1 : 2;
+    fprintf(stderr, "[driver] scale=%g poisson=%d rate=%g batch=%d light=%d\n", scale, poisson_n, rate, (int)do_batch,
+            (int)light);
 
     // ---- one simulator whose scan length (and so its FFT length and output
     // size) changes between frames, with the output shape queried before each
@@ -236,21 +240,21 @@ int main(int argc, char** argv) {
     // ---- 64 uneven jobs: per call in both projection modes, then batched
     if (do_batch) {
         std::vector<Phantom> jobs;
-        for (int j = 0; j < 64; j++) jobs.push_back(make_phantom(3, scale));
+        for (int j = 0; j < n_jobs; j++) jobs.push_back(make_phantom(3, scale));
         bcsim_batch* b;
         CHK(bcsim_batch_create(&b));
         for (auto& p : jobs) CHK(bcsim_batch_add(b, p.sim, nullptr));
         size_t total;
         int nj;
-        std::vector<size_t> offs(64);
-        std::vector<int> ls(64), ss(64);
-        CHK(bcsim_batch_get_layout(b, &total, &nj, offs.data(), ls.data(), ss.data(), 64));
+        std::vector<size_t> offs(n_jobs);
+        std::vector<int> ls(n_jobs), ss(n_jobs);
+        CHK(bcsim_batch_get_layout(b, &total, &nj, offs.data(), ls.data(), ss.data(), n_jobs));
         void* d_out = nullptr;
         hipMalloc(&d_out, total * 8);
         for (const char* mode : {"direct", "tiled", "tiled"}) {
             for (auto& p : jobs) CHK(bcsim_set_parameter(p.sim, "projection_mode", mode));
-            for (int rep = 0; rep < 3; rep++)
-                for (int j = 0; j < 64; j++) {
+            for (int rep = 0; rep < (light ? 1 : 3); rep++)
+                for (int j = 0; j < n_jobs; j++) {
                     int L, S;
                     CHK(bcsim_simulate_lines_device(jobs[j].sim, (float2*)d_out + offs[j], (size_t)ls[j] * ss[j], &L, &S));
                     CHK(bcsim_synchronize(jobs[j].sim));
@@ -343,7 +347,7 @@ int main(int argc, char** argv) {
         fprintf(stderr, "[driver] poisson done (%d)\n", done);
     }
     // sampled single simulators
-    for (int k = 0; k < 10; k++) {
+    for (int k = 0; k < (light ? 3 : 10); k++) {
         auto pg = pairs[rng() % pairs.size()];
         bcsim_sim* s;
         CHK(bcsim_create("gpu", &s));
@@ -369,11 +373,11 @@ int main(int argc, char** argv) {
     fprintf(stderr, "[driver] sampled singles done\n");
     // saturated bursts
     std::vector<std::tuple<int, int, float>> reqs;
-    for (int i = 0; i < 300; i++) {
+    for (int i = 0; i < n_sat; i++) {
         auto pg = pairs[rng() % pairs.size()];
         reqs.push_back({pg.first, pg.second, (float)U(0, 0.999)});
     }
-    for (int burst = 0; burst < 2; burst++) {
+    for (int burst = 0; burst < n_bursts; burst++) {
         std::vector<Req*> w;
         for (auto& r : reqs) w.push_back(submit(std::get<0>(r), std::get<1>(r), std::get<2>(r)));
         for (auto* r : w) wait(r);

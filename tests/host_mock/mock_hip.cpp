@@ -380,7 +380,10 @@ hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned int flags) {
     return hipSuccess;
 }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
-    *ms = 0.01f + (float)(g_rng() % 1000) * 1e-3f;  // random: the tuner picks random shapes
+    // random: the tuner picks random shapes.  The caller's thread has its own
+    // engine (g_rng belongs to the executor thread, under g_q_mu).
+    thread_local std::mt19937_64 rng(777);
+    *ms = 0.01f + (float)(rng() % 1000) * 1e-3f;
     return hipSuccess;
 }
 
@@ -496,6 +499,7 @@ void check_conv(const ConvParams& p, const char* op) {
     const size_t tw = (size_t)(p.fft_len >> p.twiddle_bits) + ((size_t)1 << p.twiddle_bits);
     check(p.twiddles, tw * 8, "twiddles", op);
     check(p.iq, (size_t)p.num_lines * p.n_out * 8, "iq", op);
+    if (p.noise_counter_ptr) check(p.noise_counter_ptr, 4, "noise counter", op);
     if (p.fft_len > kMaxFftLen || (1 << p.log2_len) != p.fft_len) die(std::string(op) + ": fft_len");
     if (p.twiddle_bits != p.log2_len / 2) die(std::string(op) + ": twiddle bits");
     if (p.n_out > 0 && (long long)p.center + (long long)(p.n_out - 1) * p.decimation >= p.fft_len)
@@ -667,9 +671,11 @@ hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int 
     return hipSuccess;
 }
 
-hipError_t launch_fill_segments(const FillSeg* segs, int n, int* counters, int nc, hipStream_t s) {
+hipError_t launch_fill_segments(const FillSeg* segs, int n, int* counters, int nc, uint32_t* noise, int n_noise,
+                                hipStream_t s) {
     enqueue(s, Op{[=] {
                       check(counters, (size_t)nc * 4, "counters", "fill_segments");
+                      check(noise, (size_t)n_noise * 4, "noise counters", "fill_segments");
                       if (n) check(segs, (size_t)n * sizeof(FillSeg), "segs", "fill_segments");
                       for (int i = 0; i < n; i++) check(segs[i].p, segs[i].n * 4, "segment", "fill_segments");
                   },

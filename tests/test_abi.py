@@ -59,3 +59,28 @@ def test_cxx_facade_compiles():
     src = os.path.join(ROOT, "tests", "cxx", "facade_check.cpp")
     res = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I", INCLUDE, src], capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_c_header_compiles_as_c99_and_links():
+    """include/bcsim_c.h is a C header: a C99 caller (tests/cxx/c_caller.c) compiles
+    with warnings as errors and links against the built library."""
+    import shutil
+    import subprocess
+    import tempfile
+    from openbcsim_amd import _lib
+    from openbcsim_amd.build import C_CALLER_FLAGS
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc is not available")
+    lib_dir = os.path.dirname(_lib.load()._name)
+    src = os.path.join(ROOT, "tests", "cxx", "c_caller.c")
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "c_caller")
+        res = subprocess.run(["gcc", *C_CALLER_FLAGS, "-I", INCLUDE, src, "-o", exe, "-L", lib_dir, "-lbcsim_hip",
+                              "-lm"], capture_output=True, text=True)
+        assert res.returncode == 0, res.stderr
+        # the header alone, in every C dialect a binding generator may use
+        for std in ("-std=c89", "-std=c11"):
+            res = subprocess.run(["gcc", std, "-Wall", "-Werror", "-pedantic", "-fsyntax-only", "-x", "c", "-I",
+                                  INCLUDE, "-"], input='#include "bcsim_c.h"\nint main(void) { return 0; }\n',
+                                 capture_output=True, text=True)
+            assert res.returncode == 0, (std, res.stderr)

@@ -47,3 +47,32 @@ def test_cxx_facade_program_matches_oracle(tmp_path):
     err = rel_l2(iq, orc.simulate_frame())
     print(f"[result] facade rel_l2={err:.3e}", flush=True)
     assert err <= 1e-4
+
+
+def test_c99_caller_matches_oracle(tmp_path):
+    """tests/cxx/c_caller.c, a C99 program over include/bcsim_c.h."""
+    exe = os.path.join(ROOT, "openbcsim_amd", "_build", "c_caller")
+    assert os.path.exists(exe), "build the C caller first (openbcsim_amd.build.build_cxx_check)"
+    out = tmp_path / "frame.bin"
+    print("[case] c_caller", flush=True)
+    res = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    raw = out.read_bytes()
+    L, S, N = (int(x) for x in np.frombuffer(raw[:12], np.int32))
+    f = np.frombuffer(raw[12:], np.float32)
+    sizes = [3 * L, 3 * L, 3 * L, L, 4 * N, 55]
+    parts, pos = [], 0
+    for n in sizes:
+        parts.append(f[pos:pos + n])
+        pos += n
+    o, d, l, t, xyza, exc = parts
+    iq = f[pos:pos + 2 * L * S].view(np.complex64).reshape(L, S)
+    orc = Oracle()
+    orc.set_parameter("radial_decimation", "2")
+    orc.set_excitation(exc, 27, 50e6, 2.5e6)
+    orc.set_scan_sequence(o.reshape(L, 3), d.reshape(L, 3), 0.04, l.reshape(L, 3), t)
+    orc.add_fixed_scatterers(xyza.reshape(N, 4))
+    orc.set_analytical_beam_profile(1e-3, 1e-3)
+    err = rel_l2(iq, orc.simulate_frame())
+    print(f"[result] c_caller rel_l2={err:.3e}", flush=True)
+    assert err <= 1e-4

@@ -84,3 +84,27 @@ def test_noise_parameter_round_trip():
     s, _ = noise_sim(0.25, seed=3)
     assert abs(float(s.get_parameter("noise_amplitude")) - 0.25) < 1e-7
     assert s.get_parameter("noise_seed") == "3"
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batched_frames_advance_the_noise_stream(use_graph):
+    """A batched frame draws the noise of the simulator's own next frame: two
+    consecutive batched frames (the second a graph replay when graphs are on)
+    differ, and equal frames 1 and 2 of an identical simulator run alone; a frame
+    of the simulator alone after them continues its stream."""
+    from openbcsim_amd.batch import Batch
+    alone, _ = noise_sim(0.8, seed=21, scatterers=2000)
+    batched, _ = noise_sim(0.8, seed=21, scatterers=2000)
+    other, _ = noise_sim(0.3, seed=5, scatterers=1000, lines=32)  # a second job in the batch
+    a = [alone.simulate_frame() for _ in range(3)]
+    b = Batch([batched, other], use_graph=use_graph)
+    b1 = b.simulate()[0]
+    b2 = b.simulate()[0]
+    e1, e2 = rel_l2(b1, a[0]), rel_l2(b2, a[1])
+    print(f"[result] graph={use_graph}: batched frame 1 vs alone 1 {e1:.2e}, 2 vs 2 {e2:.2e}, "
+          f"1 vs 2 {rel_l2(b1, b2):.2f}", flush=True)
+    assert e1 < 1e-5 and e2 < 1e-5
+    assert rel_l2(b1, b2) > 0.5, "the second batched frame draws new noise"
+    b.close()
+    b3 = batched.simulate_frame()  # frame 3 of the stream, alone
+    assert rel_l2(b3, a[2]) < 1e-5
