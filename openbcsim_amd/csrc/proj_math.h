@@ -27,9 +27,20 @@ __device__ __forceinline__ Local local_coords(const LineGeom& L, float px, float
     return o;
 }
 
-// GaussianBeamProfile::sampleProfile (BeamProfile.cpp:49-51).
-__device__ __forceinline__ float gauss_weight(const ProfileParams& P, float l, float e) {
-    return expf(-(l * l * P.inv_tsl + e * e * P.inv_tse));
+// Work done per thread, for the launch's statistics: profile-weight evaluations
+// (LUT record gathers, Gaussian exponentials) and LDS bin additions.
+struct Tally {
+    int evals = 0;
+    int adds = 0;
+};
+
+// GaussianBeamProfile::sampleProfile (BeamProfile.cpp:49-51); zero beyond the
+// weight cutoff (exponent above g_cut).
+__device__ __forceinline__ float gauss_weight(const ProfileParams& P, float l, float e, Tally& t) {
+    const float x = l * l * P.inv_tsl + e * e * P.inv_tse;
+    if (x > P.g_cut) return 0.0f;
+    t.evals++;
+    return expf(-x);
 }
 
 // LUTBeamProfile::sampleProfile (BeamProfile.cpp:79-130): truncating indices, zero
@@ -47,6 +58,7 @@ struct LutRadial {
     bool low;  // r in (-1, 0) steps: the upper radial neighbour is r0 itself
     float fr;
     bool ok;
+    int4 box;  // with a weight cutoff: records of slice r0 that can reach it (ProfileParams::slice_box)
 };
 
 __device__ __forceinline__ LutRadial lut_radial(const ProfileParams& P, float r) {
@@ -57,6 +69,7 @@ __device__ __forceinline__ LutRadial lut_radial(const ProfileParams& P, float r)
     R.off = r0 * (P.nl * P.ne);
     R.low = tr < 0.0f;
     R.fr = tr - (float)r0;
+    R.box = P.slice_box ? P.slice_box[r0] : make_int4(0, 0, 0, 0);
     return R;
 }
 
@@ -65,11 +78,15 @@ __device__ __forceinline__ float lerp(float a, float b, float f) {
     return a + f * (b - a);
 }
 
-__device__ __forceinline__ float lut_weight_le(const ProfileParams& P, const LutRadial& R, float l, float e) {
+__device__ __forceinline__ float lut_weight_le(const ProfileParams& P, const LutRadial& R, float l, float e,
+                                               Tally& t) {
     const float tl = (l - P.l_min) * P.inv_dl;
     const float te = (e - P.e_min) * P.inv_de;
     if (!(R.ok && tl > -1.0f && tl < (float)(P.nl - 1) && te > -1.0f && te < (float)(P.ne - 1))) return 0.0f;
     const int l0 = (int)tl, e0 = (int)te;
+    // weight cutoff: a record outside the slice's box holds only samples below it
+    if (P.slice_box && (l0 < R.box.x || l0 > R.box.y || e0 < R.box.z || e0 > R.box.w)) return 0.0f;
+    t.evals++;
     const float fl = tl - (float)l0, fe = te - (float)e0;
     const float4* cell = P.quad + 2 * (R.off + l0 * P.ne + e0);
     float4 a = cell[0];
@@ -92,11 +109,12 @@ __device__ __forceinline__ float lut_weight_le(const ProfileParams& P, const Lut
     const float c11 = lerp(a.w, b.w, R.fr);
     const float c0 = lerp(c00, c10, fl);
     const float c1 = lerp(c01, c11, fl);
-    return lerp(c0, c1, fe);
+    const float w = lerp(c0, c1, fe);
+    return fabsf(w) < P.w_cut ? 0.0f : w;  // w_cut is 0 without a weight cutoff
 }
 
-__device__ __forceinline__ float lut_weight(const ProfileParams& P, float r, float l, float e) {
-    return lut_weight_le(P, lut_radial(P, r), l, e);
+__device__ __forceinline__ float lut_weight(const ProfileParams& P, float r, float l, float e, Tally& t) {
+    return lut_weight_le(P, lut_radial(P, r), l, e, t);
 }
 
 // Bin of radial distance r: floor(r*scale/denom + 0.5) evaluated in double as the
@@ -123,7 +141,8 @@ template <bool PHASE, bool LUT>
 __device__ __forceinline__ void accumulate(const ProjCommon& c, float r, float l, float e, int bin,
                                            double true_index, float amp, float* lbins) {
     if (bin < 0) return;
-    const float w = (LUT ? lut_weight(c.prof, r, l, e) : gauss_weight(c.prof, l, e)) * amp;
+    Tally t;
+    const float w = (LUT ? lut_weight(c.prof, r, l, e, t) : gauss_weight(c.prof, l, e, t)) * amp;
     if (w == 0.0f) return;  // adding +-0 leaves the bin unchanged
     if (PHASE) {
         // CpuAlgorithm.cpp:85-92: w * exp(i * 2 pi f_demod * (bin - true_index) / fs)

@@ -99,8 +99,9 @@ __device__ __forceinline__ bool chunk_hits_shared(const CullParams& cp, const Li
 // Weight times amplitude added at window-relative bin lb (CpuAlgorithm.cpp:77-95).
 template <bool PHASE>
 __device__ __forceinline__ void add_bin(const ProjCommon& c, float* lbins, int lb, int bin, double true_index,
-                                        float w) {
+                                        float w, Tally& t) {
     if (w == 0.0f) return;  // adding +-0 leaves the bin unchanged
+    t.adds += PHASE ? 2 : 1;
     if (PHASE) {
         const float ph = (float)((double)bin - true_index) * c.phase_per_sample;
         float s, co;
@@ -115,8 +116,9 @@ __device__ __forceinline__ void add_bin(const ProjCommon& c, float* lbins, int l
 // add_bin with the sub-sample rotation (cos, sin) of the phase already computed
 // (the same for every line when the lines share an origin).
 template <bool PHASE>
-__device__ __forceinline__ void add_bin_rot(float* lbins, int lb, float w, float co, float s) {
+__device__ __forceinline__ void add_bin_rot(float* lbins, int lb, float w, float co, float s, Tally& t) {
     if (w == 0.0f) return;
+    t.adds += PHASE ? 2 : 1;
     if (PHASE) {
         atomicAdd(&lbins[2 * lb], w * co);
         atomicAdd(&lbins[2 * lb + 1], w * s);
@@ -136,11 +138,30 @@ __device__ __forceinline__ LineDirs line_dirs(const LineGeom& L) {
 
 template <bool PHASE, bool LUT>
 __device__ __forceinline__ void emit(const ProjCommon& c, float* lbins, int b0, int nb, float r, float l, float e,
-                                     int bin, double ti, float amp) {
+                                     int bin, double ti, float amp, Tally& t) {
     const int lb = bin - b0;
     if (bin < 0 || lb < 0 || lb >= nb) return;
-    const float w = (LUT ? lut_weight(c.prof, r, l, e) : gauss_weight(c.prof, l, e)) * amp;
-    add_bin<PHASE>(c, lbins, lb, bin, ti, w);
+    const float w = (LUT ? lut_weight(c.prof, r, l, e, t) : gauss_weight(c.prof, l, e, t)) * amp;
+    add_bin<PHASE>(c, lbins, lb, bin, ti, w, t);
+}
+
+// With a LUT weight cutoff: narrow the lateral and elevational support of the
+// cull to the records of slices [s0, s1] that hold samples above the cutoff
+// (the union of their boxes, from the sparse table), in metres with a record of
+// slack below and two above.  Returns false when none of those slices has such
+// a record, so no pair of the chunk can contribute.
+__device__ __forceinline__ bool narrow_support(const ProfileParams& P, CullParams& cp, int s0, int s1) {
+    const int n = s1 - s0 + 1;
+    const int k = 31 - __builtin_clz(n);  // largest power of two not above n
+    const int4 a = P.slice_box[k * P.nr + s0];
+    const int4 b = P.slice_box[k * P.nr + s1 - (1 << k) + 1];
+    const int l_lo = min(a.x, b.x), l_hi = max(a.y, b.y), e_lo = min(a.z, b.z), e_hi = max(a.w, b.w);
+    if (l_lo > l_hi || e_lo > e_hi) return false;
+    cp.l_lo = fmaxf(cp.l_lo, P.l_min + (float)(l_lo - 1) * cp.lut_dl);
+    cp.l_hi = fminf(cp.l_hi, P.l_min + (float)(l_hi + 2) * cp.lut_dl);
+    cp.e_lo = fmaxf(cp.e_lo, P.e_min + (float)(e_lo - 1) * cp.lut_de);
+    cp.e_hi = fminf(cp.e_hi, P.e_min + (float)(e_hi + 2) * cp.lut_de);
+    return true;
 }
 
 // One (tile, window, slice) block of work: everything a workgroup of the tile
@@ -198,6 +219,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     // drains hand its waves long lists.
     const int step = min((int)blockDim.x, g.list_cap);
     int hits = 0;  // (chunk, line) pairs this thread listed
+    Tally tally;
     __syncthreads();
     // Tiles of up to kRegLines lines sharing an origin keep the origin and line
     // directions in registers for the whole kernel: the per-hit loop then reads
@@ -234,9 +256,21 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     const float rmin = fmaxf(0.0f, dn - hn) - m, rmax = dn + hn + m;
                     const bool pos_ok = rmax > r_lo && rmin < r_hi;
                     const bool neg_ok = -rmin > r_lo && -rmax < r_hi;
-                    if (pos_ok || neg_ok) {
+                    CullParams cpc = cp;
+                    bool any = pos_ok || neg_ok;
+                    if (LUT && any && c.prof.slice_box && cp.lut_dl > 0.0f && cp.lut_de > 0.0f) {
+                        // the records the chunk's distances can use: slices of r in
+                        // [-rmax, -rmin] and/or [rmin, rmax], one slice of slack each way
+                        const ProfileParams& P = c.prof;
+                        const float lo = neg_ok ? -rmax : rmin, hi = pos_ok ? rmax : -rmin;
+                        const float top = (float)(P.nr - 2);
+                        const float t0 = fminf(fmaxf(floorf((lo - P.r_min) * P.inv_dr) - 1.0f, 0.0f), top);
+                        const float t1 = fminf(fmaxf(floorf((hi - P.r_min) * P.inv_dr) + 1.0f, 0.0f), top);
+                        any = narrow_support(P, cpc, (int)t0, (int)t1);
+                    }
+                    if (any) {
                         for (int k = 0; k < nl; k++)
-                            if (chunk_hits_shared<LUT>(cp, tl[k], B, dx, dy, dz, m, pos_ok, neg_ok)) mask |= 1u << k;
+                            if (chunk_hits_shared<LUT>(cpc, tl[k], B, dx, dy, dz, m, pos_ok, neg_ok)) mask |= 1u << k;
                     }
                 }
             } else {
@@ -308,7 +342,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     const Local q = local_coords<ARC>(tl[k], px, py, pz);
                     double ti;
                     const int bin = radial_bin(c, q.r, ti);
-                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, amp);
+                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, amp, tally);
                 }
             } else if (SHARED && ARC) {
                 const float4 p = fs.pts[s];
@@ -330,12 +364,13 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     const float e = dx * ex + dy * ey + dz * ez;
                     if (!signbit(rp)) {
                         if (!in_pos) return;
-                        const float w = (LUT ? lut_weight_le(c.prof, R, l, e) : gauss_weight(c.prof, l, e)) * p.w;
-                        add_bin_rot<PHASE>(bins + k * stride, lb_pos, w, co_pos, s_pos);
+                        const float w =
+                            (LUT ? lut_weight_le(c.prof, R, l, e, tally) : gauss_weight(c.prof, l, e, tally)) * p.w;
+                        add_bin_rot<PHASE>(bins + k * stride, lb_pos, w, co_pos, s_pos, tally);
                     } else {
                         double ti;
                         const int bin = radial_bin(c, -absd, ti);
-                        emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w);
+                        emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w, tally);
                     }
                 };
                 if (reg_lines) {
@@ -363,7 +398,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     const Local q = local_coords<ARC>(tl[k], p.x, p.y, p.z);
                     double ti;
                     const int bin = radial_bin(c, q.r, ti);
-                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, p.w);
+                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, p.w, tally);
                 }
             }
         }
@@ -375,6 +410,17 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
         // the block's work, for the order of the next launch: one atomic per wave
         for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
         if (lane == 0 && hits) atomicAdd(&g.work[wg], hits);
+    }
+    if (g.stats) {
+        unsigned long long ev = (unsigned)tally.evals, ad = (unsigned)tally.adds;
+        for (int off = 32; off > 0; off >>= 1) {
+            ev += __shfl_xor(ev, off);
+            ad += __shfl_xor(ad, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&g.stats[0], ev);
+            atomicAdd(&g.stats[1], ad);
+        }
     }
     __syncthreads();
     // flush: line k's window [b0, b0 + nb) of the global bins [line][num_rf]

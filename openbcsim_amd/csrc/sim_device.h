@@ -27,6 +27,11 @@ constexpr int kMaxFftLen = 16384;     // one line in 128 KiB of LDS
 constexpr int kDefaultRadix = 8;       // FFT pass radix when not set or tuned (8 measured faster at every length)
 constexpr int kMinFftLen = 16;
 constexpr int kLdsBytes = 160 * 1024; // gfx950 LDS per workgroup
+// Default weight cutoff (parameter weight_cutoff): profile weights below 1e-8 of
+// the peak are dropped, 6.07 sigma for a Gaussian.  A dropped pair changes its
+// bin by less than 1e-8 of a full-weight echo of the same amplitude, below the
+// rounding of a float32 sum (6e-8 relative).
+constexpr float kDefaultWeightCutoff = 1e-8f;
 
 struct alignas(16) LineGeom {
     float ox, oy, oz, t;
@@ -45,14 +50,24 @@ struct alignas(16) SplineLineBasis {
 enum ProfileKind { kProfileGaussian = 0, kProfileLut = 1 };
 
 struct ProfileParams {
-    // Gaussian: exp(-(l*l*inv_tsl + e*e*inv_tse)), inv_ts* = 1/(2 sigma^2)
+    // Gaussian: exp(-(l*l*inv_tsl + e*e*inv_tse)), inv_ts* = 1/(2 sigma^2); a pair whose
+    // exponent exceeds g_cut (-ln of the weight cutoff; +inf without one) weighs 0
     float inv_tsl, inv_tse;
+    float g_cut;
     // LUT (BeamProfile.cpp:79-130): t = (x - x_min) * inv_dx, trilinear, zero outside;
     // read as quad pairs (see lut_weight): quad[2*((r*nl + l)*ne + e) + s] holds slice r + s
     const float4* quad;
     int nr, nl, ne;
     float r_min, l_min, e_min;
     float inv_dr, inv_dl, inv_de;
+    // Weight cutoff (null / 0 without one): per radial record index r, the
+    // inclusive ranges (l_lo, l_hi, e_lo, e_hi) of the records (r, l, e) holding a
+    // sample of magnitude >= w_cut; a pair whose record lies outside weighs 0 (its
+    // weight, a convex combination of smaller samples, is below w_cut), and so
+    // does a pair whose interpolated |weight| < w_cut.  Level k of the sparse table
+    // slice_box + k*nr holds the union over records [r, r + 2^k).
+    const int4* slice_box;
+    float w_cut;
 };
 
 struct ProjCommon {
@@ -153,6 +168,7 @@ struct CullParams {
     float l_lo, l_hi;         // LUT lateral support
     float e_lo, e_hi;         // LUT elevational support
     float g_a, g_b, g_max;    // Gaussian: a*l^2 + b*e^2 < g_max
+    float lut_dl, lut_de;     // LUT grid spacing, 0 for a degenerate table (weight cutoff: records to metres)
 };
 
 struct TileLaunch {
@@ -167,6 +183,9 @@ struct TileLaunch {
     const int* order;         // workgroup w runs block order[w] (heaviest first), or null: block w
     int* work;                // per-block hit counts added up by the launch, or null
     const int* chunks;        // chunk visiting order (a permutation of [0, nchunks)), or null: index order
+    // or null: per launch, [0] profile-weight evaluations (LUT record gathers /
+    // Gaussian exponentials) and [1] LDS bin additions, added up by the launch
+    unsigned long long* stats;
 };
 
 // Blocks one order_tiles launch orders (one workgroup walks them three times).

@@ -302,6 +302,10 @@ struct bcsim_sim {
     int num_streams = 2;
     int threads_per_block = 128;
     bool store_details = false;
+    // Beam-profile weights below this fraction of the profile's peak magnitude
+    // (1 for the Gaussian, the largest |sample| of a LUT) count as zero, so their
+    // pairs are culled; 0 keeps every pair whose weight is not exactly zero.
+    float weight_cutoff = kDefaultWeightCutoff;
     uint64_t version = 0;      // bumped by every setter; batches rebuild their plans on change
     bool direct_mode = false;  // projection_mode: "tiled" (culled tile kernel) or "direct"
     // launch shape of the tile kernel set by the user; 0 = tuned or heuristic
@@ -342,6 +346,10 @@ struct bcsim_sim {
     int profile = -1;
     float sigma_l = 0.f, sigma_e = 0.f;
     std::shared_ptr<DevBuf> lut;  // float4 quad pairs [n_rad][n_lat][n_ele][2]; replaced, never rewritten
+    std::shared_ptr<const std::vector<float>> lut_samples;  // the table as given, for the cutoff's boxes
+    float lut_peak = 0.0f;                                   // largest |sample|
+    std::shared_ptr<DevBuf> lut_boxes;                       // ProfileParams::slice_box for lut_boxes_cut
+    float lut_boxes_cut = -1.0f;
     int lut_nr = 0, lut_nl = 0, lut_ne = 0;
     float lut_r0 = 0, lut_r1 = 0, lut_l0 = 0, lut_l1 = 0, lut_e0 = 0, lut_e1 = 0;
 
@@ -350,6 +358,7 @@ struct bcsim_sim {
     int num_rf = 0, fft_len = 0, log2_len = 0, n_out = 0;
     int tables_fft = 0, tables_out = 0;  // frame shape the convolution tables were built for
     DevBuf d_lines, d_H, d_demod, d_tw, d_bins, d_iq;
+    DevBuf d_stats;  // with store_kernel_details: per projection launch, weight evaluations and LDS additions
 
     std::map<std::string, std::vector<double>> debug;
 
@@ -422,6 +431,10 @@ struct bcsim_sim {
             threads_per_block = n;  // launch shapes come from the tuner; kept for compatibility
         } else if (key == "store_kernel_details") {
             store_details = parse_bool(value);
+        } else if (key == "weight_cutoff") {
+            const float v = parse_float(value);
+            if (!(v >= 0.0f && v < 1.0f)) fail(BCSIM_ERR_INVALID_ARGUMENT, "weight_cutoff must lie in [0, 1)");
+            weight_cutoff = v;
         } else if (key == "projection_mode") {
             if (value == "tiled") direct_mode = false;
             else if (value == "direct") direct_mode = true;
@@ -474,6 +487,11 @@ struct bcsim_sim {
         if (key == "threads_per_block") return std::to_string(threads_per_block);
         if (key == "store_kernel_details") return store_details ? "on" : "off";
         if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
+        if (key == "weight_cutoff") {
+            char b[32];
+            std::snprintf(b, sizeof b, "%.9g", (double)weight_cutoff);
+            return b;
+        }
         if (key == "autotune") return autotune ? "on" : "off";
         if (key == "tile_order") return tile_order ? "on" : "off";
         if (key == "chunk_order") return chunk_order ? "on" : "off";
@@ -558,6 +576,7 @@ struct bcsim_sim {
         tune = src.tune;
         autotune = src.autotune;
         radix = src.radix;
+        weight_cutoff = src.weight_cutoff;
         have_exc = src.have_exc;
         exc = src.exc;
         center = src.center;
@@ -575,6 +594,10 @@ struct bcsim_sim {
         sigma_l = src.sigma_l;
         sigma_e = src.sigma_e;
         lut = src.lut;
+        lut_samples = src.lut_samples;
+        lut_peak = src.lut_peak;
+        lut_boxes = src.lut_boxes;
+        lut_boxes_cut = src.lut_boxes_cut;
         lut_nr = src.lut_nr, lut_nl = src.lut_nl, lut_ne = src.lut_ne;
         lut_r0 = src.lut_r0, lut_r1 = src.lut_r1, lut_l0 = src.lut_l0;
         lut_l1 = src.lut_l1, lut_e0 = src.lut_e0, lut_e1 = src.lut_e1;
@@ -764,6 +787,11 @@ struct bcsim_sim {
         hip_check(hipMemcpy(table->p, quad.data(), quad.size() * sizeof(float4), hipMemcpyHostToDevice),
                   "hipMemcpy");
         lut = std::move(table);
+        lut_samples = std::make_shared<const std::vector<float>>(samples, samples + cells);
+        lut_peak = 0.0f;
+        for (size_t i = 0; i < cells; i++) lut_peak = std::max(lut_peak, std::fabs(samples[i]));
+        lut_boxes.reset();
+        lut_boxes_cut = -1.0f;
         lut_nr = nr;
         lut_nl = nl;
         lut_ne = ne;
@@ -928,10 +956,61 @@ struct bcsim_sim {
             upload_spline_basis();
             dirty_basis = false;
         }
+        if (profile == kProfileLut && weight_cutoff > 0.0f && (!lut_boxes || lut_boxes_cut != weight_cutoff)) {
+            quiesce();
+            build_slice_boxes();
+        }
         const size_t L = lines.size();
         const size_t bin_bytes = L * (size_t)num_rf * (phase_delay ? 8 : 4);
         if (bin_bytes > d_bins.bytes) quiesce();
         d_bins.ensure(bin_bytes);
+    }
+
+    // Sparse table of the LUT's per-slice record boxes for the weight cutoff (see
+    // ProfileParams::slice_box): level 0 holds, for each radial record index r, the
+    // ranges of (l, e) whose record -- samples (r..r+1, l..l+1, e..e+1), the quad
+    // pair lut_weight reads -- has a sample of magnitude >= cutoff * peak; level k
+    // the union over records [r, r + 2^k).
+    void build_slice_boxes() {
+        const int nr = lut_nr, nl = lut_nl, ne = lut_ne;
+        const std::vector<float>& v = *lut_samples;
+        const float thr = weight_cutoff * lut_peak;
+        const int kEmpty = 1 << 29;
+        std::vector<uint8_t> big((size_t)nr * nl * ne);
+        for (size_t i = 0; i < big.size(); i++) big[i] = std::fabs(v[i]) >= thr;
+        auto at = [&](int r, int l, int e) -> bool {
+            return l < nl && e < ne && big[((size_t)r * nl + l) * ne + e];
+        };
+        int levels = 1;
+        while ((1 << levels) <= nr) levels++;
+        std::vector<int4> st((size_t)levels * nr);
+        for (int r = 0; r < nr; r++) {
+            const int r1 = r + 1 < nr ? r + 1 : r;
+            int4 b = make_int4(kEmpty, -1, kEmpty, -1);
+            for (int l = 0; l < nl; l++)
+                for (int e = 0; e < ne; e++) {
+                    bool any = false;
+                    for (int q : {r, r1})
+                        any = any || at(q, l, e) || at(q, l, e + 1) || at(q, l + 1, e) || at(q, l + 1, e + 1);
+                    if (!any) continue;
+                    b.x = std::min(b.x, l), b.y = std::max(b.y, l);
+                    b.z = std::min(b.z, e), b.w = std::max(b.w, e);
+                }
+            st[r] = b;
+        }
+        for (int k = 1; k < levels; k++)
+            for (int r = 0; r < nr; r++) {
+                const int4 a = st[(size_t)(k - 1) * nr + r];
+                const int r2 = r + (1 << (k - 1));
+                const int4 c = r2 < nr ? st[(size_t)(k - 1) * nr + r2] : a;
+                st[(size_t)k * nr + r] = make_int4(std::min(a.x, c.x), std::max(a.y, c.y), std::min(a.z, c.z),
+                                                   std::max(a.w, c.w));
+            }
+        auto buf = std::make_shared<DevBuf>();
+        buf->ensure(st.size() * sizeof(int4));
+        hip_check(hipMemcpy(buf->p, st.data(), st.size() * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy");
+        lut_boxes = std::move(buf);
+        lut_boxes_cut = weight_cutoff;
     }
 
     ProjCommon proj_common(bool spline_rule) const {
@@ -955,7 +1034,12 @@ struct bcsim_sim {
         if (profile == kProfileGaussian) {
             P.inv_tsl = (float)(1.0 / (2.0 * (double)sigma_l * (double)sigma_l));
             P.inv_tse = (float)(1.0 / (2.0 * (double)sigma_e * (double)sigma_e));
+            P.g_cut = weight_cutoff > 0.0f ? (float)(-std::log((double)weight_cutoff)) : INFINITY;
         } else {
+            P.g_cut = INFINITY;
+            const bool cut = weight_cutoff > 0.0f && lut_boxes && lut_boxes_cut == weight_cutoff;
+            P.slice_box = cut ? lut_boxes->as<int4>() : nullptr;
+            P.w_cut = cut ? weight_cutoff * lut_peak : 0.0f;
             P.quad = lut->as<float4>();
             P.nr = lut_nr;
             P.nl = lut_nl;
@@ -1017,7 +1101,23 @@ struct bcsim_sim {
     };
     std::vector<Pending> pending;
 
+    size_t stats_launches = 0;  // projection launches of the last frame that filled d_stats
+
     void collect_timings() {
+        if (stats_launches) {
+            // the launches' own counts (TileLaunch::stats): the weight evaluations and
+            // LDS additions of each projection launch
+            std::vector<unsigned long long> st(2 * stats_launches);
+            hip_check(hipMemcpyAsync(st.data(), d_stats.p, st.size() * sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, stream),
+                      "hipMemcpyAsync");
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            for (size_t i = 0; i < stats_launches; i++) {
+                debug["projection_weight_evals"].push_back((double)st[2 * i]);
+                debug["projection_lds_adds"].push_back((double)st[2 * i + 1]);
+            }
+            stats_launches = 0;
+        }
         for (auto& p : pending) {
             float ms = 0.f;
             if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
@@ -1050,18 +1150,25 @@ struct bcsim_sim {
             cp.l_hi = (float)((double)lut_l0 + (lut_nl - 1) * std::fabs(dl));
             cp.e_lo = (float)((double)lut_e0 - std::fabs(de));
             cp.e_hi = (float)((double)lut_e0 + (lut_ne - 1) * std::fabs(de));
+            cp.lut_dl = (float)dl;
+            cp.lut_de = (float)de;
             if (!(dr > 0 && dl > 0 && de > 0)) {  // degenerate table: no culling
                 r_lo = -INFINITY;
                 r_hi = INFINITY;
                 cp.l_lo = cp.e_lo = -INFINITY;
                 cp.l_hi = cp.e_hi = INFINITY;
+                cp.lut_dl = cp.lut_de = 0.0f;  // and no narrowing by the weight cutoff's boxes
             }
         } else {
             // expf(-x) rounds to zero for x > 103.98 (below half the smallest denormal);
-            // 110 leaves a 6 % margin on the exponent
+            // 110 leaves a 6 % margin on the exponent.  With a weight cutoff the pairs
+            // beyond g_cut weigh zero; the cull keeps a relative 1e-4 above it, far
+            // more than the rounding of a pair's exponent.
             cp.g_a = c.prof.inv_tsl;
             cp.g_b = c.prof.inv_tse;
-            cp.g_max = (std::isfinite(cp.g_a) && std::isfinite(cp.g_b)) ? 110.0f : INFINITY;
+            float gm = 110.0f;
+            if (std::isfinite(c.prof.g_cut)) gm = std::min(gm, c.prof.g_cut * 1.0001f);
+            cp.g_max = (std::isfinite(cp.g_a) && std::isfinite(cp.g_b)) ? gm : INFINITY;
         }
         cp.r_per_bin = (float)per_sample;
         cp.r_lo = (float)r_lo;
@@ -1386,6 +1493,12 @@ struct bcsim_sim {
         } else if (store_details) {
             debug["kernel_memset_ms"].push_back(0.0);  // every bin is stored by the projection
         }
+        stats_launches = 0;
+        if (store_details && !direct_mode && !targets.empty()) {
+            d_stats.ensure(2 * sizeof(unsigned long long) * targets.size());
+            hip_check(launch_fill_zero(d_stats.as<float>(), 4 * targets.size(), stream), "fill kernel");
+            stats_launches = targets.size();
+        }
         for (size_t i = 0; i < targets.size(); i++) {
             const ProjTarget& t = targets[i];
             const bool is_spline = std::strcmp(t.kind, "fixed") != 0;
@@ -1402,6 +1515,7 @@ struct bcsim_sim {
             } else {
                 TileLaunch cfg = plans[i];
                 cfg.atomic |= single ? 0 : 1;
+                if (stats_launches) cfg.stats = d_stats.as<unsigned long long>() + 2 * i;
                 const int blocks = tile_blocks(cfg);
                 if (tile_order && blocks >= kMinOrderedBlocks && blocks <= kMaxOrderedTiles) {
                     int* wo = order_buffer(i, blocks);

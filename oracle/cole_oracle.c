@@ -76,6 +76,7 @@ struct oracle_sim {
     int phase_delay;
     int decimation;
     int sum_all_cs;
+    double weight_cutoff; /* product parameter; 0 (the default) is the reference's sum */
     /* excitation */
     int have_exc;
     float* exc;
@@ -99,6 +100,7 @@ struct oracle_sim {
     float lut_r0, lut_r1, lut_l0, lut_l1, lut_e0, lut_e1;
     int lut_nr, lut_nl, lut_ne;
     float* lut;
+    double lut_peak; /* max |sample|, the reference magnitude of the weight cutoff */
     char err[512];
 };
 
@@ -150,6 +152,13 @@ int oracle_set_parameter(oracle_sim* s, const char* key, const char* value) {
         int d = atoi(value);
         if (d <= 0) return fail(s, "illegal radial decimation value");
         s->decimation = d;
+    } else if (!strcmp(key, "weight_cutoff")) {
+        /* The product's cull of weights below this fraction of the profile's peak
+         * (1 for the Gaussian, max |sample| of a LUT).  Not a reference parameter:
+         * it lets the tests pin the product's rule; the default 0 keeps every pair. */
+        const double c = strtod(value, NULL);
+        if (!(c >= 0.0 && c < 1.0)) return fail(s, "weight_cutoff must lie in [0, 1)");
+        s->weight_cutoff = c;
     } else if (!strcmp(key, "noise_amplitude")) {
         if (strtof(value, NULL) != 0.0f) return fail(s, "oracle evaluates noise-free frames only");
     } else if (!strcmp(key, "verbose")) {
@@ -295,6 +304,8 @@ int oracle_set_lut_profile(oracle_sim* s, float r_min, float r_max, float l_min,
     free(s->lut);
     s->lut = (float*)malloc(sizeof(float) * tot);
     memcpy(s->lut, samples, sizeof(float) * tot);
+    s->lut_peak = 0.0;
+    for (size_t i = 0; i < tot; i++) s->lut_peak = fmax(s->lut_peak, fabs((double)samples[i]));
     s->lut_r0 = r_min; s->lut_r1 = r_max;
     s->lut_l0 = l_min; s->lut_l1 = l_max;
     s->lut_e0 = e_min; s->lut_e1 = e_max;
@@ -508,15 +519,21 @@ double oracle_lut_sample(oracle_sim* s, double r, double l, double e) {
     return (1.0 - fe) * c0 + fe * c1;
 }
 
+/* Beam-profile weight; with a weight cutoff c, weights below c times the
+ * profile's peak magnitude (1 for the Gaussian, the LUT's max |sample|) are zero. */
 static double profile_weight(oracle_sim* s, float r, float l, float e) {
     if (s->profile == PROFILE_GAUSSIAN) {
         /* BeamProfile.cpp:49-56 */
         const double tsl = 2.0 * (double)s->sigma_l * (double)s->sigma_l;
         const double tse = 2.0 * (double)s->sigma_e * (double)s->sigma_e;
         const double ld = l, ed = e;
-        return exp(-(ld * ld / tsl + ed * ed / tse));
+        const double x = ld * ld / tsl + ed * ed / tse;
+        if (s->weight_cutoff > 0.0 && x > -log(s->weight_cutoff)) return 0.0;
+        return exp(-x);
     }
-    return oracle_lut_sample(s, (double)r, (double)l, (double)e);
+    const double w = oracle_lut_sample(s, (double)r, (double)l, (double)e);
+    if (s->weight_cutoff > 0.0 && fabs(w) < s->weight_cutoff * s->lut_peak) return 0.0;
+    return w;
 }
 
 /* ---------------------------------------------------------------- projection */

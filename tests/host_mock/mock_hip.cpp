@@ -432,6 +432,12 @@ constexpr size_t kLds = 160 * 1024;
 void check_common(const ProjCommon& c, bool lut, const char* op) {
     check(c.lines, (size_t)c.num_lines * sizeof(LineGeom), "lines", op);
     if (lut) check(c.prof.quad, (size_t)c.prof.nr * c.prof.nl * c.prof.ne * 2 * sizeof(float4), "lut", op);
+    if (lut && c.prof.slice_box) {
+        // every level of the sparse table: nr entries each, levels while 2^k <= nr
+        int levels = 1;
+        while ((1 << levels) <= c.prof.nr) levels++;
+        check(c.prof.slice_box, (size_t)levels * c.prof.nr * sizeof(int4), "slice boxes", op);
+    }
 }
 
 void check_spline(const SplineSet& s, int num_lines, bool basis, const char* op) {
@@ -573,6 +579,7 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams&, const Fix
     enqueue(s, Op{[=] {
                       check_tile(c, is_spline ? nullptr : &fs, is_spline ? &ss : nullptr, boxes, nchunks,
                                  (const float*)bins, phase, lut, cfg, cfg.block, "project_tile");
+                      if (cfg.stats) check(cfg.stats, 2 * sizeof(unsigned long long), "stats", "project_tile");
                   },
                   "project_tile"});
     return hipSuccess;
