@@ -145,6 +145,27 @@ BCSIM_API bcsim_status bcsim_simulate_lines(bcsim_sim* sim, float* out, size_t c
 BCSIM_API bcsim_status bcsim_simulate_lines_device(bcsim_sim* sim, void* d_out, size_t capacity,
                                                    int* num_lines, int* num_samples);
 BCSIM_API bcsim_status bcsim_synchronize(bcsim_sim* sim);
+
+/* Allocation-free variant of bcsim_simulate_lines_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a frame
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or other allocating routine: the
+ * frame's projection bins (and the rendered points of all-timestamps-equal
+ * spline sets) live in a fixed scratch array of the library, 32 MiB per GPU, held
+ * until the process ends and shared by every call on that GPU, which keeps the
+ * last call's contents.  Calls that overlap on one GPU (concurrent calls from
+ * several handles or threads, or frames still in flight on other streams) race on
+ * it: serialise them.  Frames needing more scratch than 32 MiB (L * num_rf * 4
+ * bytes of bins, 8 with phase delay, plus 16 bytes per rendered spline) fail with
+ * BCSIM_ERR_CAPACITY.  The handle's tables must already be built for its current
+ * configuration: after configuring it (or changing its configuration) run one
+ * frame through bcsim_simulate_lines_device or bcsim_simulate_lines, else
+ * BCSIM_ERR_INVALID_STATE.  Launch shapes are the user-set or heuristic ones (no
+ * tuning trials), blocks run in index order, and store_kernel_details records
+ * nothing; the IQ equals bcsim_simulate_lines_device's up to float summation
+ * order.  bcsim_debug_get_projection fails after such a frame. */
+BCSIM_API bcsim_status bcsim_simulate_lines_device_noalloc(bcsim_sim* sim, void* d_out, size_t capacity,
+                                                           int* num_lines, int* num_samples);
 /* The handle's hipStream_t, as void*. */
 BCSIM_API bcsim_status bcsim_get_stream(bcsim_sim* sim, void** stream);
 

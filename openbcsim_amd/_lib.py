@@ -38,6 +38,7 @@ SIGNATURES = {
     "bcsim_get_output_shape": [_h, _ip, _ip],
     "bcsim_simulate_lines": [_h, _fp, C.c_size_t, _ip, _ip],
     "bcsim_simulate_lines_device": [_h, C.c_void_p, C.c_size_t, _ip, _ip],
+    "bcsim_simulate_lines_device_noalloc": [_h, C.c_void_p, C.c_size_t, _ip, _ip],
     "bcsim_synchronize": [_h],
     "bcsim_get_stream": [_h, C.POINTER(C.c_void_p)],
     "bcsim_get_debug_data": [_h, C.c_char_p, _dp, C.c_size_t, _sp],

@@ -223,6 +223,16 @@ struct OrderSeg {
     int slices;
 };
 
+// Scratch of the allocation-free frame (bcsim_simulate_lines_device_noalloc): a
+// fixed device array of this many bytes per GPU in sim_kernels.hip, holding one
+// frame's projection bins and rendered spline points.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr int kMaxScratchDevices = 64;
+
+// Device address of the scratch array on `device` (the current device), looked
+// up once per device with hipGetSymbolAddress and kept.
+hipError_t noalloc_scratch(int device, void** ptr);
+
 // Launchers (sim_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_fill_zero(float* p, size_t n_floats, hipStream_t stream);
 hipError_t launch_project_fixed(const ProjCommon& c, const FixedSet& s, void* bins, bool complex_bins,

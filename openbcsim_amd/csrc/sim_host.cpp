@@ -182,6 +182,15 @@ BCSIM_API bcsim_status bcsim_simulate_lines_device(bcsim_sim* sim, void* d_out, 
     });
 }
 
+BCSIM_API bcsim_status bcsim_simulate_lines_device_noalloc(bcsim_sim* sim, void* d_out, size_t capacity,
+                                                           int* num_lines, int* num_samples) {
+    return guarded([&] {
+        need(sim);
+        if (!d_out) fail(BCSIM_ERR_INVALID_ARGUMENT, "null output buffer");
+        sim->simulate_device_noalloc(d_out, capacity, num_lines, num_samples);
+    });
+}
+
 BCSIM_API bcsim_status bcsim_synchronize(bcsim_sim* sim) {
     return guarded([&] {
         need(sim);

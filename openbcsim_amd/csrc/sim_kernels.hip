@@ -22,10 +22,37 @@
 // they select a bin.  The FFT code opts back into contraction locally.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "proj_math.h"
 #include "sim_device.h"
 
 namespace bcsim_hip {
+
+// ------------------------------------------------------------------------------------
+// Scratch of the allocation-free frame entry point: one frame's bins and rendered
+// spline points live here instead of in buffers of the simulator.  Every GPU the
+// library runs on holds its own copy until the process ends; host code never
+// passes the array's name to a kernel (in host code it is only a stand-in), but
+// the device address looked up below for the current device.
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+    static std::mutex mu;
+    static void* addr[kMaxScratchDevices] = {};
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
 
 // ------------------------------------------------------------------------------------
 // utilities

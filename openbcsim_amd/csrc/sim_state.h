@@ -94,11 +94,12 @@ struct ChunkOrders {
     }
 
     // Device array of chunk indices (a permutation of [0, nchunks)), or null.
-    const int* get(const Key& key) {
+    // create = false: only an order built earlier (no allocation).
+    const int* get(const Key& key, bool create = true) {
         std::lock_guard<std::mutex> g(mu);
         for (auto& e : orders)
             if (e.first == key) return e.second->as<int>();
-        if (orders.size() >= kMaxOrders || centres.empty()) return nullptr;
+        if (!create || orders.size() >= kMaxOrders || centres.empty()) return nullptr;
         std::vector<std::pair<float, int>> k(centres.size());
         for (size_t c = 0; c < centres.size(); c++) {
             const float4 q = centres[c];
@@ -379,7 +380,12 @@ struct bcsim_sim {
 
     void use_device() {
         hip_check(hipSetDevice(device), "hipSetDevice");
-        if (!stream) hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        if (!stream) {
+            hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+            // set-up of the device: the address of its allocation-free scratch
+            void* scratch_addr = nullptr;
+            hip_check(noalloc_scratch(device, &scratch_addr), "hipGetSymbolAddress(scratch)");
+        }
         can_change_device = false;
     }
 
@@ -1252,9 +1258,35 @@ struct bcsim_sim {
         return ChunkOrders::Key{m.ox, m.oy, m.oz, m.rx, m.ry, m.rz, 0.f};
     }
 
-    std::vector<ProjTarget> proj_targets() {
+    // Where an allocation-free frame keeps what a frame otherwise holds in buffers
+    // of the simulator: the bins, then the rendered points of all-timestamps-equal
+    // spline sets, carved from the fixed scratch array (256-byte aligned pieces).
+    struct FrameMem {
+        unsigned char* scratch = nullptr;
+        size_t used = 0, cap = 0;
+        float* bins = nullptr;
+        void* carve(size_t bytes) {
+            const size_t at = (used + 255) & ~(size_t)255;
+            if (at + bytes > cap) fail(BCSIM_ERR_CAPACITY, "allocation-free frame: scratch exhausted");
+            used = at + bytes;
+            return scratch + at;
+        }
+    };
+
+    // Bytes of scratch an allocation-free frame of the current configuration needs.
+    size_t noalloc_bytes() const {
+        size_t b = (lines.size() * (size_t)num_rf * (phase_delay ? 8 : 4) + 255) & ~(size_t)255;
+        if (all_equal)
+            for (auto& sp : splines) b += (sp->n * sizeof(float4) + 255) & ~(size_t)255;
+        return b;
+    }
+
+    // The frame's projection launches.  fm (allocation-free frames): rendered
+    // points go to its scratch and only chunk orders built earlier are used.
+    std::vector<ProjTarget> proj_targets(FrameMem* fm = nullptr) {
         std::vector<ProjTarget> out;
         const ChunkOrders::Key okey = chunk_order_key();
+        const bool create = fm == nullptr;
         for (auto& f : fixed) {
             if (f->n == 0) continue;
             ProjTarget t{"fixed", proj_common(false)};
@@ -1262,7 +1294,7 @@ struct bcsim_sim {
             t.boxes = f->boxes.as<ChunkBox>();
             t.nchunks = f->nchunks;
             t.n = f->n;
-            if (chunk_order) t.chunk_order = f->visit.get(okey);
+            if (chunk_order) t.chunk_order = f->visit.get(okey, create);
             out.push_back(t);
         }
         for (size_t q = 0; q < splines.size(); q++) {
@@ -1273,15 +1305,21 @@ struct bcsim_sim {
             t.ss = SplineSet{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
                              scan->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
             if (all_equal) {
-                scan->rendered.ensure(sp->n * sizeof(float4));
-                t.fs = FixedSet{scan->rendered.as<float4>(), (int)sp->n};
+                float4* dst;
+                if (fm) {
+                    dst = static_cast<float4*>(fm->carve(sp->n * sizeof(float4)));
+                } else {
+                    scan->rendered.ensure(sp->n * sizeof(float4));
+                    dst = scan->rendered.as<float4>();
+                }
+                t.fs = FixedSet{dst, (int)sp->n};
             }
             t.boxes = sp->boxes.as<ChunkBox>();
             t.nchunks = sp->nchunks;
             t.n = sp->n;
             t.sp = sp;
             t.scan = scan;
-            if (chunk_order) t.chunk_order = sp->visit.get(okey);
+            if (chunk_order) t.chunk_order = sp->visit.get(okey, create);
             out.push_back(t);
         }
         return out;
@@ -1289,9 +1327,9 @@ struct bcsim_sim {
 
     // Render the all-timestamps-equal spline sets at line 0's time
     // (GpuAlgorithm.cpp:181-187, GpuScatterers.cpp:48-123).
-    void enqueue_renders() {
+    void enqueue_renders(const std::vector<ProjTarget>& targets) {
         if (!all_equal) return;
-        for (const ProjTarget& t : proj_targets()) {
+        for (const ProjTarget& t : targets) {
             Timed tm(this, "kernel_render_ms");
             render_target(t);
             tm.stop();
@@ -1302,7 +1340,7 @@ struct bcsim_sim {
         if (std::strcmp(t.kind, "rendered") != 0) return;
         SplineLineBasis B0;
         host_basis_line0(*t.sp, B0);
-        hip_check(launch_render_spline(t.ss, B0, t.scan->rendered.as<float4>(), stream), "render kernel");
+        hip_check(launch_render_spline(t.ss, B0, const_cast<float4*>(t.fs.pts), stream), "render kernel");
     }
 
     // Blocks (tile, window) of one slice of a tile launch.
@@ -1451,8 +1489,11 @@ struct bcsim_sim {
             debug[k].push_back(0.0);
     }
 
-    void enqueue_frame(float2* d_out) {
-        prepare();
+    // Enqueue one frame.  fm: an allocation-free frame (its bins and rendered points
+    // in the fixed scratch array; heuristic or user-set launch shapes, no tuning
+    // trials, no block-order buffers, no timing events).
+    void enqueue_frame(float2* d_out, FrameMem* fm = nullptr) {
+        if (!fm) prepare();
         if (store_details) {
             debug.clear();
             record_fused_keys();
@@ -1460,11 +1501,11 @@ struct bcsim_sim {
         const bool cplx = phase_delay;
         const size_t L = lines.size();
         const size_t bin_floats = L * (size_t)num_rf * (cplx ? 2 : 1);
-        float* bins = d_bins.as<float>();
+        float* bins = fm ? fm->bins : d_bins.as<float>();
         const bool lut_profile = profile == kProfileLut;
 
-        enqueue_renders();
-        const std::vector<ProjTarget> targets = proj_targets();
+        const std::vector<ProjTarget> targets = proj_targets(fm);
+        enqueue_renders(targets);
         // plan every launch first: tuning trials write the bins
         std::vector<TileLaunch> plans;
         std::vector<ProjLaunch> direct_plans;
@@ -1473,14 +1514,14 @@ struct bcsim_sim {
             if (direct_mode) {
                 direct_plans.push_back(proj_launch(t.n));
             } else {
-                plans.push_back(plan_tile(t, autotune));
+                plans.push_back(plan_tile(t, autotune && !fm));
                 const TileLaunch& g = plans.back();
                 last_plan += std::string(t.kind) + ": lines=" + std::to_string(g.tile_lines) +
                              " block=" + std::to_string(g.block) + " slices=" + std::to_string(g.slices) +
                              " windows=" + std::to_string(g.windows) + "\n";
             }
         }
-        const int frame_radix = plan_radix(autotune);
+        const int frame_radix = plan_radix(autotune && !fm);
         last_plan += "convolve: radix=" + std::to_string(frame_radix) + "\n";
 
         // The bins are zeroed first unless a single launch writes every bin with plain stores.
@@ -1517,7 +1558,7 @@ struct bcsim_sim {
                 cfg.atomic |= single ? 0 : 1;
                 if (stats_launches) cfg.stats = d_stats.as<unsigned long long>() + 2 * i;
                 const int blocks = tile_blocks(cfg);
-                if (tile_order && blocks >= kMinOrderedBlocks && blocks <= kMaxOrderedTiles) {
+                if (!fm && tile_order && blocks >= kMinOrderedBlocks && blocks <= kMaxOrderedTiles) {
                     int* wo = order_buffer(i, blocks);
                     cfg.work = wo;
                     cfg.order = wo + blocks;
@@ -1529,6 +1570,7 @@ struct bcsim_sim {
             tm.stop();
         }
         ConvParams p = conv_params(d_out);
+        p.bins = bins;
         p.radix = frame_radix;
         {
             Timed t(this, "kernel_convolve_demodulate_ms", "kernel_forward_fft_ms");
@@ -1584,6 +1626,7 @@ struct bcsim_sim {
         if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
         if (nl) *nl = (int)lines.size();
         if (ns) *ns = n_out;
+        last_frame_noalloc = false;
         enqueue_frame(static_cast<float2*>(d_out));
         if (store_details) {
             debug["kernel_memcpy_ms"].push_back(0.0);  // the IQ stays in device memory
@@ -1591,8 +1634,59 @@ struct bcsim_sim {
         }
     }
 
+    bool last_frame_noalloc = false;  // the last frame's bins were in the shared scratch
+
+    // simulate_device without allocating: the bins (and rendered spline points) of
+    // the frame live in the library's fixed scratch array of this GPU, which every
+    // allocation-free frame on the GPU shares.  Everything else a frame reads must
+    // already exist: the simulator's tables are built by its first frame through
+    // simulate_device or simulate_host, and a configuration change needs such a
+    // frame again.  No launch-shape tuning, block ordering or kernel timing.
+    void simulate_device_noalloc(void* d_out, size_t capacity, int* nl, int* ns) {
+        require_configured();
+        if (lines.empty()) fail(BCSIM_ERR_INVALID_STATE, "No scanlines in scansequence");
+        if (profile < 0) fail(BCSIM_ERR_INVALID_STATE, "No beam profile is configured");
+        const bool lut_ready = profile != kProfileLut || weight_cutoff <= 0.0f ||
+                               (lut_boxes && lut_boxes_cut == weight_cutoff);
+        bool basis_ready = true;
+        if (!all_equal)
+            for (auto& sc : spline_scan) basis_ready = basis_ready && sc->basis.bytes >= lines.size() * sizeof(SplineLineBasis);
+        if (!stream || dirty_scan || dirty_conv || dirty_basis || !lut_ready || !basis_ready || fft_len != tables_fft ||
+            n_out != tables_out || d_lines.bytes < lines.size() * sizeof(LineGeom))
+            fail(BCSIM_ERR_INVALID_STATE,
+                 "allocation-free frame: the simulator's tables are not built for its current configuration; "
+                 "run one frame through bcsim_simulate_lines_device or bcsim_simulate_lines first");
+        const size_t need = lines.size() * (size_t)n_out;
+        if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
+        const size_t bytes = noalloc_bytes();
+        if (bytes > kNoAllocScratchBytes)
+            fail(BCSIM_ERR_CAPACITY, "allocation-free frame needs " + std::to_string(bytes) +
+                                         " bytes of scratch (bins and rendered points), more than its fixed " +
+                                         std::to_string(kNoAllocScratchBytes) + "; use bcsim_simulate_lines_device");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        void* base = nullptr;
+        hip_check(noalloc_scratch(device, &base), "hipGetSymbolAddress(scratch)");
+        FrameMem fm;
+        fm.scratch = static_cast<unsigned char*>(base);
+        fm.cap = kNoAllocScratchBytes;
+        fm.bins = static_cast<float*>(fm.carve(lines.size() * (size_t)num_rf * (phase_delay ? 8 : 4)));
+        const bool details = store_details;
+        store_details = false;  // timing events are created per frame
+        try {
+            enqueue_frame(static_cast<float2*>(d_out), &fm);
+        } catch (...) {
+            store_details = details;
+            throw;
+        }
+        store_details = details;
+        last_frame_noalloc = true;
+        if (nl) *nl = (int)lines.size();
+        if (ns) *ns = n_out;
+    }
+
     void simulate_host(float* out, size_t capacity, int* nl, int* ns) {
         prepare();
+        last_frame_noalloc = false;
         const size_t need = lines.size() * (size_t)n_out;
         if (capacity < need) fail(BCSIM_ERR_CAPACITY, "output capacity too small");
         d_iq.ensure(need * sizeof(float2));
@@ -1648,6 +1742,8 @@ struct bcsim_sim {
 
     void debug_projection(float* out, size_t capacity, int* nl, int* nrf) {
         if (!d_bins.p || dirty_scan) fail(BCSIM_ERR_INVALID_STATE, "no simulated frame");
+        if (last_frame_noalloc)
+            fail(BCSIM_ERR_INVALID_STATE, "the last frame ran without allocation: its bins were in the shared scratch");
         const size_t L = lines.size();
         if (nl) *nl = (int)L;
         if (nrf) *nrf = num_rf;

@@ -211,6 +211,13 @@ int main(int argc, char** argv) {
             CHK(bcsim_get_output_shape(f.sim, &L, &S));
             std::vector<float> host((size_t)L * S * 2);
             CHK(bcsim_simulate_lines(f.sim, host.data(), (size_t)L * S, &L, &S));
+            // allocation-free frames after the tables exist (bins in the shared scratch)
+            void* d_out = nullptr;
+            hipMalloc(&d_out, (size_t)L * S * 8);
+            for (int i = 0; i < 2; i++)
+                CHK(bcsim_simulate_lines_device_noalloc(f.sim, d_out, (size_t)L * S, &L, &S));
+            CHK(bcsim_synchronize(f.sim));
+            hipFree(d_out);
         }
         CHK(bcsim_destroy(f.sim));
         fprintf(stderr, "[driver] scan-length changes done\n");

@@ -678,6 +678,18 @@ hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int 
     return hipSuccess;
 }
 
+// The library's fixed scratch array of the allocation-free frame: one mock device
+// allocation, looked up like hipGetSymbolAddress would.
+hipError_t noalloc_scratch(int device, void** ptr) {
+    static std::mutex mu;
+    static void* addr = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr && hipMalloc(&addr, kNoAllocScratchBytes) != hipSuccess) return hipErrorOutOfMemory;
+    *ptr = addr;
+    return hipSuccess;
+}
+
 hipError_t launch_fill_segments(const FillSeg* segs, int n, int* counters, int nc, uint32_t* noise, int n_noise,
                                 hipStream_t s) {
     enqueue(s, Op{[=] {
