@@ -5,7 +5,7 @@
 A step is one frame of the hot path (projection + fused convolution/demodulation)
 over the workload's resident phantom; the timed region holds K frames enqueued
 back to back on the simulator's stream with the IQ written to device memory,
-bracketed by device synchronisation.  Prints one JSON line.
+bracketed by device synchronisation.  Prints one JSON line (progress on stderr).
 
 Workload (default): the 3D volume scan quoted "at 1M scatterers" -- 1M fixed
 float32 points in a 6x6x11 cm box, 64x64 = 4096 pyramidal lines over +-30 deg,
@@ -13,26 +13,41 @@ a 256x64x64 float32 LUT beam profile, FFT length 16384, decimation 2.  Others:
 spline_mmode, fixed_sector_2d, and each name with "_phase" appended (phase delay on).
 
 roofline: the projection kernel's average duration is measured with HIP events
-on the simulator's stream (store_kernel_details) over the same K frames; its
-algorithmic bytes are 16 B per scatterer evaluation (x, y, z, amplitude read
-once per line in the per-line streaming model) times L*N evaluations per launch.
+on the simulator's stream (store_kernel_details) over K frames.  `traffic` is its
+HBM bytes per launch from two rocprofv3 --pmc runs of this script in a child
+process (FETCH_SIZE, then WRITE_SIZE), each scaled by what the same counter reads
+for a plain copy kernel of known size; achieved = traffic / duration against the
+8 TB/s HBM peak.  Beside it, the units that bind the kernel: its own counts of
+profile-weight evaluations (LUT record gathers) and LDS float additions per
+launch, as rates against ceilings measured on the device by kernels of the same
+access shape (openbcsim_amd/ceilings.py), and the per-line streaming model of the
+reference's kernel (16 B per scatterer evaluation) as a ratio to the peak.
 
 cpu_baseline: the reference's own CPU algorithm (CpuAlgorithm, compiled from the
 reference sources into oracle/_ref) on a bounded sample of the same workload --
-the first lines of the scan against all scatterers -- with all host cores.
+the first lines of the scan against all scatterers -- with all host cores: the
+median of 5 runs after a warm-up.  The same leg checks the sampled served
+requests against the FP64 oracle (a few lines of each).
 
 Further legs in the same JSON line (each can be skipped): `configs` (the 2D sector,
-the spline M-mode and the volume, phase delay off and on), `batched_64_jobs` (64
-uneven jobs per call vs batched) and `serving` (Poisson arrivals against 16 resident
-phantoms: p50/p99 latency; saturated throughput against per-call loops).
+the spline M-mode and the volume, phase delay off and on, each also with the
+weight cutoff off), `batched_64_jobs` (64 uneven jobs per call vs batched),
+`small_jobs` (256 small jobs, where launches dominate) and `serving` (Poisson
+arrivals against 16 resident phantoms: p50/p99 latency; saturated throughput
+against per-call loops).
 """
 from __future__ import annotations
 
 import argparse
+import csv
 import ctypes as C
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -42,11 +57,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_EVAL = {"fixed": 16, "spline": 52}
-# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0, the scales a 256 MiB device copy reads back as), for the committed kernels;
-# profiles/project_tile_volume_lut_1M_4096_pmc.md gives the commands and counts.
-MEASURED_TRAFFIC = {"volume_lut": {"bytes": 1.867e6 * 1024 * 2.0 + 1.573e5 * 1024,
-                                   "source": "profiles/project_tile_volume_lut_1M_4096_pmc.md"}}
+COPY_CALIBRATION_BYTES = 256 << 20
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def build_workload(name: str):
@@ -86,7 +101,7 @@ def run_gpu(wl, steps: int, warmup: int, tile=None):
     ms_per_step = (t1 - t0) * 1e3 / steps
     # kernel durations with HIP events on the simulator's stream, same frames again
     sim.set_parameter("store_kernel_details", "on")
-    kern = {}
+    kern, counts = {}, {}
     for _ in range(steps):
         sim.simulate_lines_device(out.address, L * S)
         for key in ("fixed_projection_kernel_ms", "spline_projection_kernel_ms", "kernel_convolve_demodulate_ms",
@@ -95,6 +110,9 @@ def run_gpu(wl, steps: int, warmup: int, tile=None):
                 kern.setdefault(key, []).extend(sim.get_debug_data(key))
             except RuntimeError:
                 pass
+        # the projection launch's own counts (the same every frame)
+        for key in ("projection_weight_evals", "projection_lds_adds"):
+            counts[key] = float(sum(sim.get_debug_data(key)))
     sim.set_parameter("store_kernel_details", "off")
     # PCIe-inclusive rate (host output), reported beside the device-resident value
     sim.simulate_frame()
@@ -105,14 +123,15 @@ def run_gpu(wl, steps: int, warmup: int, tile=None):
     host_ms = (t3 - t2) * 1e3 / max(1, steps // 2)
     sim.close()
     out.close()
-    return L, S, ms_per_step, {k: float(np.mean(v)) for k, v in kern.items() if v}, host_ms
+    return L, S, ms_per_step, {k: float(np.mean(v)) for k, v in kern.items() if v}, host_ms, counts
 
 
 def run_configs(steps: int = 5):
     """Every single-simulator config with phase delay off and on (device-resident IQ,
     frame time over `steps` frames after one warm-up): the 2D sector (100k points,
     128 lines), the spline M-mode (200k cubic splines, 256 lines, one time per
-    line) and the 1M-point volume."""
+    line) and the 1M-point volume; each again with weight_cutoff 0 (every pair
+    whose weight is not exactly zero, as the reference sums them)."""
     from openbcsim_amd import workloads as W
     from openbcsim_amd.hip_rt import DeviceBuffer
     from openbcsim_amd.pyrfsim import RfSimulator
@@ -125,30 +144,58 @@ def run_configs(steps: int = 5):
             wl.apply(sim)
             L, S = sim.output_shape()
             buf = DeviceBuffer(L * S * 8)
-            sim.simulate_lines_device(buf.address, L * S)
-            sim.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
+            res = {}
+            for cutoff in (None, "0"):
+                if cutoff is not None:
+                    sim.set_parameter("weight_cutoff", cutoff)
                 sim.simulate_lines_device(buf.address, L * S)
-            sim.synchronize()
-            dt = (time.perf_counter() - t0) / steps
+                sim.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    sim.simulate_lines_device(buf.address, L * S)
+                sim.synchronize()
+                res[cutoff] = (time.perf_counter() - t0) / steps
+            dt = res[None]
             out[f"{name}{'_phase' if phase else ''}"] = {
                 "lines": L, "scatterers": wl.num_scatterers, "ms_per_frame": round(dt * 1e3, 4),
-                "iq_lines_per_s": round(L / dt, 1), "scatterer_evals_per_s": round(L * wl.num_scatterers / dt, 1)}
+                "iq_lines_per_s": round(L / dt, 1), "scatterer_evals_per_s": round(L * wl.num_scatterers / dt, 1),
+                "ms_per_frame_cutoff_off": round(res["0"] * 1e3, 4),
+                "iq_lines_per_s_cutoff_off": round(L / res["0"], 1)}
+            log(f"config {name}{'_phase' if phase else ''}: {dt * 1e3:.4f} ms per frame "
+                f"({res['0'] * 1e3:.4f} without the weight cutoff)")
             sim.close()
             buf.close()
     return out
 
 
-def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
+def small_jobs(num_jobs: int = 256, seed: int = 9):
+    """Many small simulations, where per-call launch and synchronisation costs
+    dominate: 32-64 sector lines, 10k-50k fixed points or cubic splines, lines of
+    2.8 or 6 cm (FFT 2048 or 4096), Gaussian profile."""
+    from openbcsim_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(num_jobs):
+        n = int(rng.integers(10_000, 50_001))
+        lines = int(rng.integers(32, 65))
+        length = float(rng.choice([0.028, 0.06]))
+        sub_seed = int(rng.integers(0, 1 << 30))
+        if rng.integers(0, 2):
+            jobs.append(W.spline_mmode(n, lines, seed=sub_seed, line_length=length, decimation=2))
+        else:
+            jobs.append(W.fixed_sector_2d(n, lines, seed=sub_seed, line_length=length, decimation=2))
+    return jobs
+
+
+def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3, jobs=None, direct: bool = True):
     """The 64 uneven jobs (scatterer counts log-uniform in [10k, 2M], fixed and spline,
-    32-512 lines, FFT 2048-16384, analytic or LUT): one frame of every job
+    32-512 lines, FFT 2048-16384, analytic or LUT), or `jobs`: one frame of every job
       - per_call_direct: the plain per-call GPU path -- one synchronous frame per job
         (the reference's call pattern) with the reference's projection method, one
         thread per (scatterer, line) pair accumulating with global atomics
         (projection_mode "direct"), and this library's fused convolution;
       - per_call: one synchronous frame per job with the culled tile kernel and the
-        heuristic launch shapes;
+        heuristic launch shapes (this library's own per-call path: the batching figure);
       - batched: one batched frame (tuned shapes, persistent launches, graph replay).
     Every leg writes device-resident IQ."""
     from openbcsim_amd import workloads as W
@@ -156,7 +203,9 @@ def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
     from openbcsim_amd.hip_rt import DeviceBuffer, hip
     from openbcsim_amd.pyrfsim import RfSimulator
 
-    jobs = W.batched_jobs(num_jobs, seed=3, scale=scale)
+    if jobs is None:
+        jobs = W.batched_jobs(num_jobs, seed=3, scale=scale)
+    num_jobs = len(jobs)
     sims = []
     for wl in jobs:
         s = RfSimulator("gpu")
@@ -184,7 +233,8 @@ def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
             s.set_parameter("projection_mode", mode)
 
     res = {}
-    for name, fn in (("per_call_direct", per_call), ("per_call", per_call), ("batched", batched)):
+    legs = (("per_call_direct", per_call),) if direct else ()
+    for name, fn in legs + (("per_call", per_call), ("batched", batched)):
         set_mode("direct" if name == "per_call_direct" else "tiled")
         fn()
         fn()
@@ -198,15 +248,18 @@ def run_batched(num_jobs: int = 64, scale: float = 1.0, reps: int = 3):
     b.close()
     for s in sims:
         s.close()
-    return {"jobs": num_jobs, "lines_per_frame": lines, "scatterer_evals_per_frame": scat,
-            "per_call_direct_ms": round(res["per_call_direct"] * 1e3, 3),
-            "per_call_ms": round(res["per_call"] * 1e3, 3), "batched_ms": round(res["batched"] * 1e3, 3),
-            "per_call_direct_lines_per_s": round(lines / res["per_call_direct"], 1),
-            "per_call_lines_per_s": round(lines / res["per_call"], 1),
-            "batched_lines_per_s": round(lines / res["batched"], 1),
-            "speedup_vs_plain_per_call": round(res["per_call_direct"] / res["batched"], 3),
-            "speedup_vs_tiled_per_call": round(res["per_call"] / res["batched"], 3),
-            "plan_and_tune_s": round(tune_s, 3)}
+    out = {"jobs": num_jobs, "lines_per_frame": lines, "scatterer_evals_per_frame": scat,
+           "per_call_ms": round(res["per_call"] * 1e3, 3), "batched_ms": round(res["batched"] * 1e3, 3),
+           "per_call_lines_per_s": round(lines / res["per_call"], 1),
+           "batched_lines_per_s": round(lines / res["batched"], 1),
+           "speedup_vs_tiled_per_call": round(res["per_call"] / res["batched"], 3),
+           "plan_and_tune_s": round(tune_s, 3)}
+    if direct:
+        out.update({"per_call_direct_ms": round(res["per_call_direct"] * 1e3, 3),
+                    "per_call_direct_lines_per_s": round(lines / res["per_call_direct"], 1),
+                    "speedup_vs_plain_per_call": round(res["per_call_direct"] / res["batched"], 3)})
+    log(f"batch of {num_jobs} jobs: per-call {res['per_call'] * 1e3:.3f} ms, batched {res['batched'] * 1e3:.3f} ms")
+    return out
 
 
 def run_serving(seconds: float = 60.0, rate: float = 200.0, scale: float = 1.0, seed: int = 11,
@@ -240,21 +293,26 @@ def run_serving(seconds: float = 60.0, rate: float = 200.0, scale: float = 1.0, 
     for h in warm:
         srv.wait(h)
     warm_s = time.perf_counter() - t0
+    log(f"serving: {len(pairs)} phantom/geometry pairs warm; {seconds:.0f} s of Poisson arrivals at {rate:.0f}/s")
     pinned = poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_outputs=True)
+    log(f"serving: p50 {pinned['p50_latency_ms']} ms, p99 {pinned['p99_latency_ms']} ms; numpy outputs next")
     pageable = poisson_run(srv, phantoms, geometries, pairs, rng, rate, min(seconds, compare_seconds),
                            pinned_outputs=False)
+    log("serving: saturated bursts")
     sat = saturated_serving(srv, sims, geometries, pairs, rng)
     srv.close()
     for s in sims:
         s.close()
     out = dict(pinned)
+    samples = out.pop("_samples")
+    pageable.pop("_samples")
     out.update({"outputs": "pinned (Server.alloc_output: IQ written by DMA, no copy in wait)",
                 "pageable_outputs": {k: pageable[k] for k in ("requests", "p50_latency_ms", "p99_latency_ms",
                                                               "mean_latency_ms", "latency_by_lines",
                                                               "copy_to_caller_ms", "sampled_rel_err_vs_single")},
                 "phantoms": len(phantoms), "geometries": len(geometries), "pairs": len(pairs),
                 "warmup_s": round(warm_s, 2), "saturated": sat})
-    return out
+    return out, (phantoms, geometries, samples)
 
 
 def poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_outputs):
@@ -341,7 +399,8 @@ def poisson_run(srv, phantoms, geometries, pairs, rng, rate, seconds, pinned_out
         worst = max(worst, float(np.linalg.norm(kept[i] - ref) / max(np.linalg.norm(ref), 1e-30)))
     for arr in allocated:
         srv.free_output(arr)
-    return {"requests": n, "seconds": round(wall, 3), "offered_rate_per_s": rate,
+    samples = [(pairs[int(choice[i])], float(times[i]), kept[i]) for i in sorted(sample)]
+    return {"_samples": samples, "requests": n, "seconds": round(wall, 3), "offered_rate_per_s": rate,
             "achieved_rate_per_s": round(n / wall, 1), "max_submit_lag_ms": round(late * 1e3, 2),
             "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
@@ -434,13 +493,35 @@ def saturated_serving(srv, sims, geometries, pairs, rng, n: int = 300, clients: 
     return res
 
 
-def cpu_baseline(wl, target_s: float = 12.0):
-    """Reference CPU algorithm on the first lines of the scan (all scatterers)."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def omp_max_threads() -> int:
+    """omp_get_max_threads() of the OpenMP runtime the CPU algorithm runs on."""
+    for name in ("libgomp.so.1", "libomp.so"):
+        try:
+            return int(C.CDLL(name).omp_get_max_threads())
+        except (OSError, AttributeError):
+            continue
+    return 0
+
+
+def cpu_baseline(wl, target_s: float = 20.0, reps: int = 5):
+    """Reference CPU algorithm on the first lines of the scan (all scatterers): one
+    warm-up run, then the median of `reps` runs, each about target_s / (reps + 1)
+    seconds of wall clock; configuration is outside the timed region."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from backends import Reference, ref_available, Oracle
     import copy
 
-    cores = os.cpu_count() or 1
     if ref_available():
         kind, make = "reference", Reference
     else:
@@ -454,20 +535,197 @@ def cpu_baseline(wl, target_s: float = 12.0):
         sub.apply(sim)
         t0 = time.perf_counter()
         sim.simulate_frame()
-        return time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        sim.close()
+        return dt
 
-    probe = max(1, min(cores, wl.num_lines))
-    t_probe = run(probe)
-    per_line = t_probe / probe
-    n = int(max(probe, min(wl.num_lines, (target_s / 3) / max(per_line, 1e-9))))
-    n = max(cores, (n // cores) * cores) if n >= cores else n
+    nproc = os.cpu_count() or 1
+    threads = omp_max_threads() or nproc
+    probe = max(1, min(threads, wl.num_lines))
+    per_line = run(probe) / probe
+    n = int(max(probe, min(wl.num_lines, (target_s / (reps + 1)) / max(per_line, 1e-9))))
+    n = max(threads, (n // threads) * threads) if n >= threads else n
     n = min(n, wl.num_lines)
-    times = [run(n) for _ in range(3)]
+    log(f"cpu baseline ({kind}): {n} lines per run, {threads} threads")
+    run(n)  # warm-up
+    times = [run(n) for _ in range(reps)]
     t = float(np.median(times))
-    return {"value": n / t, "unit": "IQ lines/s", "cores": cores, "kind": kind,
-            "sample": f"first {n} of {wl.num_lines} lines x all {wl.num_scatterers} scatterers, "
-                      f"median of 3 runs ({t:.2f} s each), OpenMP over {cores} host threads",
-            "scatterer_evals_per_s": n * wl.num_scatterers / t}
+    N = wl.num_scatterers
+    return {"value": n / t, "unit": "IQ lines/s", "cores": threads, "kind": kind,
+            "sample": f"first {n} of {wl.num_lines} lines x all {N} scatterers; median of {reps} runs "
+                      f"({t:.2f} s each, min {min(times):.2f}, max {max(times):.2f}) after one warm-up run; "
+                      f"OpenMP over {threads} threads",
+            "nproc": nproc, "omp_max_threads": threads, "cpu_model": cpu_model(),
+            "scatterer_evals_per_s": n * N / t, "ns_per_scatterer_per_line": t / (n * N) * 1e9}
+
+
+def served_vs_oracle(phantoms, geometries, samples, lines_per_request: int = 8):
+    """The sampled served requests against the FP64 oracle: for each, the frame of
+    its phantom with its scan at its time t (every pair, no weight cutoff), on
+    `lines_per_request` evenly spaced lines of the request."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from backends import Oracle, rel_l2
+    from openbcsim_amd import workloads as W
+    errs = []
+    for (p, g), t, out in samples:
+        wl = W.at_time(phantoms[p], geometries[g], t)
+        o, d, l, ts, length = wl.scan
+        sel = np.unique(np.linspace(0, o.shape[0] - 1, min(lines_per_request, o.shape[0])).astype(int))
+        wl.scan = (o[sel], d[sel], l[sel], ts[sel], length)
+        orc = Oracle()
+        wl.apply(orc)
+        errs.append(rel_l2(out[sel], orc.simulate_frame()))
+        orc.close()
+    return {"requests": len(errs), "lines_per_request": lines_per_request,
+            "max": float(max(errs)) if errs else None, "median": float(np.median(errs)) if errs else None}
+
+
+def pmc_child(workload: str, frames: int = 3) -> None:
+    """What the counter runs profile: a plain copy kernel of known size (the
+    calibration), then `frames` frames of the workload."""
+    from openbcsim_amd import ceilings
+    from openbcsim_amd.hip_rt import DeviceBuffer
+    from openbcsim_amd.pyrfsim import RfSimulator
+    log("counter child: copy calibration")
+    ceilings.copy_rate(COPY_CALIBRATION_BYTES)  # two dispatches of copy_kernel
+    wl = build_workload(workload)
+    sim = RfSimulator("gpu")
+    wl.apply(sim)
+    L, S = sim.output_shape()
+    out = DeviceBuffer(L * S * 8)
+    for _ in range(frames):
+        sim.simulate_lines_device(out.address, L * S)
+    sim.synchronize()
+    sim.close()
+    out.close()
+    log(f"counter child: {frames} frames of {workload} done")
+
+
+def _pmc_values(out_dir: str):
+    """{counter: {kernel name: [value per dispatch, in dispatch order]}} from every
+    rocprofv3 CSV under out_dir that holds counter values."""
+    per = {}
+    for path in glob.glob(os.path.join(out_dir, "**", "*.csv"), recursive=True):
+        with open(path, newline="") as f:
+            rows = [{k.lower().replace("-", "_"): v for k, v in r.items() if k} for r in csv.DictReader(f)]
+        if not rows or "counter_value" not in rows[0] or "kernel_name" not in rows[0]:
+            continue
+        rows.sort(key=lambda r: int(r.get("dispatch_id") or 0))
+        for r in rows:
+            per.setdefault(r.get("counter_name", "?"), {}).setdefault(r["kernel_name"], []).append(
+                float(r["counter_value"]))
+    return per
+
+
+# counter passes of measure_traffic: FETCH_SIZE takes 3 TCC counters, WRITE_SIZE 2, so
+# they run apart; the SQ instruction counts ride along with the first
+PMC_PASSES = (("FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD"), ("WRITE_SIZE",))
+
+
+def measure_traffic(workload: str, keep_dir=None):
+    """HBM bytes per launch of the projection and convolution kernels, from two
+    rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE) on
+    `workload`, each scaled by the counter's reading for copy_kernel moving
+    COPY_CALIBRATION_BYTES; with the projection's VALU, LDS and vector-memory
+    instruction counts.  Returns (dict, None) or (None, reason)."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    readings = {}
+    for counters in PMC_PASSES:
+        if keep_dir:
+            os.makedirs(keep_dir, exist_ok=True)
+        out_dir = os.path.abspath(tempfile.mkdtemp(prefix="pmc_", dir=keep_dir or tempfile.gettempdir()))
+        cmd = [exe, "--pmc", *counters, "-f", "csv", "-d", out_dir, "-o", "pmc", "--", sys.executable,
+               os.path.abspath(__file__), "--pmc-child", "--workload", workload]
+        log(f"counter run: {' '.join(counters)}")
+        try:
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240,
+                               cwd=tempfile.gettempdir(), env=dict(os.environ, TMPDIR=tempfile.gettempdir()))
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 --pmc {counters[0]} timed out"
+        with open(os.path.join(out_dir, "run.log"), "w") as f:
+            f.write(" ".join(cmd) + "\n" + r.stdout)
+        vals = _pmc_values(out_dir)
+        if counters[0] not in vals:
+            files = sorted(os.path.relpath(p, out_dir) for p in glob.glob(os.path.join(out_dir, "**"), recursive=True))
+            return None, (f"rocprofv3 --pmc {counters[0]}: no counter values (rc {r.returncode}; files {files[:12]}; "
+                          f"output head {r.stdout[:600]!r} ... tail {r.stdout[-600:]!r})")
+        readings.update(vals)
+        if not keep_dir:
+            shutil.rmtree(out_dir, ignore_errors=True)
+
+    def pick(counter, substr):
+        got = [v for k, vs in readings.get(counter, {}).items() if substr in k for v in vs]
+        return float(np.median(got)) if got else None
+
+    res = {}
+    scale = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = pick(counter, "copy_kernel")
+        if not cal:
+            return None, f"no copy_kernel dispatch in the {counter} run"
+        scale[counter] = COPY_CALIBRATION_BYTES / (cal * 1024.0)  # bytes per KiB the counter reports
+    for name, substr in (("projection", "project_tile_kernel"), ("convolution", "convolve_demod_kernel")):
+        f, w = pick("FETCH_SIZE", substr), pick("WRITE_SIZE", substr)
+        if f is None or w is None:
+            continue
+        res[name] = {"fetch_kib": f, "write_kib": w,
+                     "bytes": f * 1024.0 * scale["FETCH_SIZE"] + w * 1024.0 * scale["WRITE_SIZE"]}
+        for c in PMC_PASSES[0][1:]:
+            v = pick(c, substr)
+            if v is not None:
+                res[name][c] = v
+    res["scale"] = {"fetch": scale["FETCH_SIZE"], "write": scale["WRITE_SIZE"],
+                    "calibration": f"copy_kernel moving {COPY_CALIBRATION_BYTES >> 20} MiB read + "
+                                   f"{COPY_CALIBRATION_BYTES >> 20} MiB written"}
+    return res, None
+
+
+def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings):
+    """The dominant kernel against HBM (measured bytes) and against the measured
+    ceilings of the units its counters show busy: LDS float atomics and LUT record
+    gathers (the launch's own counts) and VALU issue (SQ_INSTS_VALU)."""
+    N = wl.num_scatterers
+    t = proj_ms * 1e-3
+    rl = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": "project_tile_kernel",
+          "kernel_ms": round(proj_ms, 4)}
+    proj = (traffic or {}).get("projection")
+    if proj:
+        b = proj["bytes"]
+        rl.update({"achieved": round(b / t / 1e9, 1), "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4),
+                   "traffic": round(b),
+                   "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this run (child process), "
+                                     "scaled by a copy kernel of known size",
+                   "traffic_detail": traffic})
+    else:
+        rl.update({"achieved": None, "frac": None, "traffic": None, "traffic_error": traffic_err})
+    model = BYTES_PER_EVAL[kind] * L * N
+    rl["streaming_model_bytes"] = model
+    rl["streaming_model_ratio"] = round(model / t / 1e9 / HBM_PEAK_GBS, 3)
+    units = {}
+    if ceilings:
+        adds, evals = (counts or {}).get("projection_lds_adds"), (counts or {}).get("projection_weight_evals")
+        if adds:
+            c = ceilings["lds_float_atomics_per_s"]
+            units["lds_float_atomics"] = {"per_launch": adds, "achieved_per_s": adds / t, "ceiling_per_s": c,
+                                          "frac": round(adds / t / c, 4)}
+        if evals and wl.profile[0] == "lut":
+            c = ceilings["lut_gathers_per_s"]
+            units["lut_record_gathers"] = {"per_launch": evals, "achieved_per_s": evals / t, "ceiling_per_s": c,
+                                           "frac": round(evals / t / c, 4)}
+        if proj and proj.get("SQ_INSTS_VALU"):
+            v, c = proj["SQ_INSTS_VALU"], ceilings["valu_wave_instructions_per_s"]
+            units["valu"] = {"per_launch": v, "achieved_per_s": v / t, "ceiling_per_s": c, "frac": round(v / t / c, 4)}
+        if proj:
+            c = ceilings["copy_bytes_per_s"]
+            units["hbm"] = {"per_launch": proj["bytes"], "achieved_per_s": proj["bytes"] / t, "ceiling_per_s": c,
+                            "frac": round(proj["bytes"] / t / c, 4)}
+        rl["units"] = units
+        rl["ceilings"] = {k: round(v, 1) for k, v in ceilings.items()}
+        if units:
+            rl["binding_unit"] = max(units, key=lambda k: units[k]["frac"])
+    return rl
 
 
 def main():
@@ -477,52 +735,46 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="volume_lut")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched-vs-per-call leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-job and small-job batched-vs-per-call legs")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scatterer-count scale of the 64 jobs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the per-config (phase off/on) leg")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure the projection's HBM traffic")
+    ap.add_argument("--pmc-dir", default=None, help="keep the counter runs' rocprofv3 output under this directory")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the timed serving run (the numpy-output comparison runs 10 s at most)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--tile", default=None, help="projection launch shape 'windows,lines,block' (default: automatic)")
-    ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="also run one device-to-device copy of this many MiB (a plain copy kernel of "
-                         "known size, to scale rocprof FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
-    if args.copy_calibration_mib > 0:
-        from openbcsim_amd.hip_rt import DeviceBuffer, hip
-        nbytes = args.copy_calibration_mib << 20
-        src, dst = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
-        hip().hipMemcpy(C.c_void_p(dst.address), C.c_void_p(src.address), nbytes, 3)
-        hip().hipDeviceSynchronize()
-        src.close()
-        dst.close()
+    if args.pmc_child:
+        pmc_child(args.workload)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(lines shard across GPUs without communication; see the multi-GPU design notes)",
               file=sys.stderr)
         sys.exit(2)
 
+    # the counter runs first, in child processes, while this process has not yet
+    # touched the GPU
+    traffic, traffic_err = (None, "skipped (--no-traffic)")
+    if not args.no_traffic and not args.tile:
+        traffic, traffic_err = measure_traffic(args.workload, args.pmc_dir)
+        log(f"traffic: {traffic if traffic else traffic_err}")
     wl = build_workload(args.workload)
-    L, S, ms, kern, host_ms = run_gpu(wl, args.steps, args.warmup, args.tile)
+    log(f"workload {args.workload}: {wl.num_lines} lines x {wl.num_scatterers} scatterers")
+    L, S, ms, kern, host_ms, counts = run_gpu(wl, args.steps, args.warmup, args.tile)
+    log(f"{ms:.4f} ms per frame; kernels {kern}")
     N = wl.num_scatterers
     evals = L * N
     kind = "fixed" if wl.fixed else "spline"
-    pkey = projection_key(wl)
-    proj_ms = kern.get(pkey)
-    roofline = None
-    if proj_ms:
-        algo_bytes = BYTES_PER_EVAL[kind] * evals
-        achieved = algo_bytes / (proj_ms * 1e-3) / 1e9
-        measured = MEASURED_TRAFFIC.get(args.workload) if not args.tile else None
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": round(measured["bytes"]) if measured else None,
-                    "kernel": pkey.replace("_ms", ""), "kernel_ms": round(proj_ms, 4),
-                    "algorithmic_bytes": algo_bytes}
-        if measured:
-            roofline["traffic_source"] = measured["source"]
-            roofline["traffic_gbs"] = round(measured["bytes"] / (proj_ms * 1e-3) / 1e9, 1)
+    proj_ms = kern.get(projection_key(wl))
+    from openbcsim_amd import ceilings as CE
+    ceil = CE.measure()
+    log(f"ceilings: {ceil}")
+    roofline = roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceil) if proj_ms else None
     line = {
         "metric": "IQ lines/s & scatterer-evals/s at 1M scatterers; serving p50/p99 frame latency",
         "value": round(L / (ms * 1e-3), 1),
@@ -538,9 +790,11 @@ def main():
         "data": "synthetic (seeded phantom, scan and LUT)",
         "config": {"workload": args.workload, "lines": L, "scatterers": N, "samples_per_line": S,
                    "fft_len": wl.fft_len(), "decimation": int(wl.params.get("radial_decimation", 1)),
-                   "profile": wl.profile[0], "phase_delay": wl.params.get("phase_delay", "off")},
+                   "profile": wl.profile[0], "phase_delay": wl.params.get("phase_delay", "off"),
+                   "weight_cutoff": 1e-8},
         "scatterer_evals_per_s": round(evals / (ms * 1e-3), 1),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+        "projection_counts": counts,
         "pcie_inclusive_lines_per_s": round(L / (host_ms * 1e-3), 1),
         "roofline": roofline,
     }
@@ -548,10 +802,15 @@ def main():
         line["configs"] = run_configs()
     if not args.no_batch:
         line["batched_64_jobs"] = run_batched(64, args.batch_scale)
+        line["small_jobs"] = run_batched(jobs=small_jobs(256), direct=False)
+    served = None
     if not args.no_serve:
-        line["serving"] = run_serving(args.serve_seconds, args.serve_rate)
+        line["serving"], served = run_serving(args.serve_seconds, args.serve_rate)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl)
+        if served:
+            log("served requests against the FP64 oracle")
+            line["serving"]["sampled_rel_err_vs_oracle"] = served_vs_oracle(*served)
     print(json.dumps(line), flush=True)
 
 

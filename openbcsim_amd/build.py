@@ -56,6 +56,18 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+CEILINGS_LIB = os.path.join(PKG, "libbcsim_ceilings.so")
+
+
+def build_ceilings() -> str:
+    """Compile the benchmark's ceiling kernels (csrc/ceilings.hip) into
+    libbcsim_ceilings.so: measurement infrastructure, not the simulation path."""
+    tmp = CEILINGS_LIB + ".tmp"
+    _run([HIPCC, *COMMON, "-shared", "-x", "hip", os.path.join(CSRC, "ceilings.hip"), "-o", tmp])
+    os.replace(tmp, CEILINGS_LIB)
+    return CEILINGS_LIB
+
+
 def build_oracle() -> None:
     """Build the test-only checkers under oracle/ (the FP64 oracle, and the compiled
     reference CPU path when the reference tree is present)."""
@@ -85,4 +97,5 @@ def build_cxx_check() -> str:
 if __name__ == "__main__":
     path = build(verbose="-v" in sys.argv)
     build_cxx_check()
+    build_ceilings()
     print(path)

@@ -78,16 +78,16 @@ class TuneCache {
     std::map<std::string, std::string> records_;
 };
 
-// Average duration in ms of `run` (which enqueues on `stream`): one untimed
-// launch, then `reps` launches between two events.
-inline double time_launches(hipStream_t stream, const std::function<void()>& run, int reps) {
+// Average duration in ms of `run` (which enqueues on `stream`): `warm` untimed
+// launches, then `reps` launches between two events.
+inline double time_launches(hipStream_t stream, const std::function<void()>& run, int reps, int warm = 2) {
     struct Ev {
         hipEvent_t e = nullptr;
         ~Ev() {
             if (e) (void)hipEventDestroy(e);
         }
     } a, b;
-    run();
+    for (int i = 0; i < warm; i++) run();
     if (hipEventCreate(&a.e) != hipSuccess || hipEventCreate(&b.e) != hipSuccess) return 0.0;
     (void)hipEventRecord(a.e, stream);
     for (int i = 0; i < reps; i++) run();

@@ -52,7 +52,16 @@ struct bcsim_batch {
     // same noise as the simulator's own consecutive frames would.
     DevBuf d_noise;
     int n_proj_items = 0, n_conv_items = 0, n_fill = 0, n_order_segs = 0;
-    size_t proj_lds = 0;
+    // Projection jobs run as one persistent launch per kernel variant (JobFlags
+    // that select code: arc, phase, LUT, spline, shared origin), each over its own
+    // contiguous range of the item table with its own counter: a kernel holding
+    // every variant needs more scalar registers than there are and spills them in
+    // its inner loops.  Groups run heaviest first.
+    struct ProjGroup {
+        int variant, first, count;
+        size_t lds;
+    };
+    std::vector<ProjGroup> groups;
     struct Render {
         SplineSet set;
         SplineLineBasis basis;
@@ -200,7 +209,6 @@ struct bcsim_batch {
         std::vector<int> job_first_target, job_targets;
         std::vector<uint32_t> noise_init;
         size_t offset = 0;
-        proj_lds = 0;
         for (bcsim_sim* s : sims) {
             s->prepare();
             JobPlan jp{s, s->version, offset, (int)s->lines.size(), s->n_out};
@@ -272,21 +280,39 @@ struct bcsim_batch {
                 fills.push_back(FillSeg{s->d_bins.as<float>(),
                                         s->lines.size() * (size_t)s->num_rf * (s->phase_delay ? 2 : 1)});
         }
-        for (const ProjJob& J : pjobs)
-            proj_lds = std::max(proj_lds, tile_lds_bytes(J.g.window_bins, (J.flags & kJobPhase) != 0, J.g.tile_lines,
-                                                         J.g.list_cap));
         total_out = offset;
 
-        // items, heaviest first
+        // items: grouped by kernel variant (groups heaviest first), jobs within a
+        // group heaviest first
+        std::vector<int> vkey(pjobs.size());
+        std::map<int, double> group_cost;
+        for (size_t j = 0; j < pjobs.size(); j++) {
+            vkey[j] = batch_variant(pjobs[j].flags);
+            group_cost[vkey[j]] += pcost[j];
+        }
         std::vector<int> porder(pjobs.size());
         std::iota(porder.begin(), porder.end(), 0);
-        std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return pcost[a] > pcost[b]; });
+        std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) {
+            if (vkey[a] != vkey[b]) {
+                const double ca = group_cost[vkey[a]], cb = group_cost[vkey[b]];
+                return ca != cb ? ca > cb : vkey[a] < vkey[b];
+            }
+            return pcost[a] > pcost[b];
+        });
         std::vector<int2> pitems;
         std::vector<size_t> first_item(pjobs.size(), 0);
+        groups.clear();
         for (int j : porder) {
+            if (groups.empty() || groups.back().variant != vkey[j])
+                groups.push_back(ProjGroup{vkey[j], (int)pitems.size(), 0, 0});
+            ProjGroup& G = groups.back();
+            const ProjJob& J = pjobs[j];
+            G.lds = std::max(G.lds, tile_lds_bytes(J.g.window_bins, (J.flags & kJobPhase) != 0, J.g.tile_lines,
+                                                   J.g.list_cap));
             first_item[j] = pitems.size();
-            const int n = pjobs[j].blocks_per_slice * pjobs[j].g.slices;
+            const int n = J.blocks_per_slice * J.g.slices;
             for (int i = 0; i < n; i++) pitems.push_back(make_int2(j, i));
+            G.count = (int)pitems.size() - G.first;
         }
         std::vector<int> corder(cjobs.size());
         std::iota(corder.begin(), corder.end(), 0);
@@ -302,7 +328,7 @@ struct bcsim_batch {
         grow(d_order_segs, ordered.size() * sizeof(OrderSeg));
         grow(d_conv_items, citems.size() * sizeof(int2));
         grow(d_conv_jobs, cjobs.size() * sizeof(ConvParams));
-        grow(d_counters, 2 * sizeof(int));
+        grow(d_counters, (groups.size() + 1) * sizeof(int));  // one per projection group, then the convolution's
         grow(d_noise, cjobs.size() * sizeof(uint32_t));
         for (size_t k = 0; k < cjobs.size(); k++) cjobs[k].noise_counter_ptr = d_noise.as<uint32_t>() + k;
         std::vector<OrderSeg> order_segs;
@@ -352,17 +378,21 @@ struct bcsim_batch {
     }
 
     void enqueue() {
-        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), 2, d_noise.as<uint32_t>(),
-                                       (int)plan.size(), stream),
+        const int ng = (int)groups.size();
+        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), ng + 1,
+                                       d_noise.as<uint32_t>(), (int)plan.size(), stream),
                   "fill kernel");
         for (const Render& r : renders)
             hip_check(launch_render_spline(r.set, r.basis, r.out, stream), "render kernel");
         hip_check(launch_order_segments(d_order_segs.as<OrderSeg>(), n_order_segs, stream), "block order kernel");
-        hip_check(launch_project_batch(d_proj_jobs.as<ProjJob>(), d_proj_items.as<int2>(), n_proj_items,
-                                       d_counters.as<int>(), proj_lds, workgroups, stream),
-                  "batched projection kernel");
+        for (int g = 0; g < ng; g++) {
+            const ProjGroup& G = groups[g];
+            hip_check(launch_project_batch(G.variant, d_proj_jobs.as<ProjJob>(), d_proj_items.as<int2>() + G.first,
+                                           G.count, d_counters.as<int>() + g, G.lds, workgroups, stream),
+                      "batched projection kernel");
+        }
         hip_check(launch_convolve_batch(d_conv_jobs.as<ConvParams>(), d_conv_items.as<int2>(), n_conv_items,
-                                        d_counters.as<int>() + 1, workgroups, stream),
+                                        d_counters.as<int>() + ng, workgroups, stream),
                   "batched convolution kernel");
     }
 

@@ -167,9 +167,8 @@ __device__ __forceinline__ bool narrow_support(const ProfileParams& P, CullParam
 // One (tile, window, slice) block of work: everything a workgroup of the tile
 // kernel does.  `wg` = tile * windows + window; `lds` is the dynamic LDS and
 // `list_n` a workgroup-shared counter.  CP_AHEAD: a spline's active control
-// points are all loaded before the first is used (the tile kernel; the
-// persistent batch kernel, which holds every variant, keeps one control point
-// at a time: measured faster there).
+// points are all loaded before the first is used (otherwise one control point
+// at a time).
 template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED, bool CP_AHEAD = true>
 __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams& cp, const FixedSet& fs,
                                            const SplineSet& ss, const ChunkBox* __restrict__ boxes, int nchunks,
@@ -529,22 +528,11 @@ hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t s
     return hipGetLastError();
 }
 
-// Persistent projection over many jobs (simulators x datasets): every
-// workgroup takes (job, block) items from a global counter until the list is
-// exhausted, and runs the block with the job's own kernel variant.
-template <bool ARC, bool PHASE, bool LUT>
-__device__ __forceinline__ void tile_block_job(const ProjJob& J, int wg, int slice, float* lds, int& list_n) {
-    if (J.flags & kJobSpline)
-        tile_block<ARC, PHASE, LUT, true, false, false>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg,
-                                                        slice, lds, list_n);
-    else if (ARC && (J.flags & kJobShared))
-        tile_block<ARC, PHASE, LUT, false, true>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg, slice,
-                                                 lds, list_n);
-    else
-        tile_block<ARC, PHASE, LUT, false, false>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg, slice,
-                                                  lds, list_n);
-}
-
+// Persistent projection over many jobs (simulators x datasets) of one kernel
+// variant: every workgroup takes (job, block) items from a global counter until
+// the list is exhausted, and runs the block.  KIND: 0 fixed points, 1 fixed
+// points with a shared origin (arc projection), 2 splines.
+template <bool ARC, bool PHASE, bool LUT, int KIND>
 __global__ __launch_bounds__(1024) void project_batch_kernel(const ProjJob* __restrict__ jobs,
                                                              const int2* __restrict__ items, int n_items,
                                                              int* __restrict__ counter) {
@@ -554,44 +542,61 @@ __global__ __launch_bounds__(1024) void project_batch_kernel(const ProjJob* __re
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = item_s;
+        const int it = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();
         if (it >= n_items) break;
         const int2 w = items[it];  // (job, block within the job)
-        const ProjJob J = jobs[w.x];
+        const ProjJob& J = jobs[__builtin_amdgcn_readfirstlane(w.x)];
+        const int blk = __builtin_amdgcn_readfirstlane(w.y);
         const int per_slice = J.blocks_per_slice;
-        const int wg = w.y % per_slice, slice = w.y / per_slice;
-        const bool arc = J.flags & kJobArc, phase = J.flags & kJobPhase, lut = J.flags & kJobLut;
-        if (arc) {
-            if (phase) {
-                if (lut) tile_block_job<true, true, true>(J, wg, slice, lds, list_n);
-                else tile_block_job<true, true, false>(J, wg, slice, lds, list_n);
-            } else {
-                if (lut) tile_block_job<true, false, true>(J, wg, slice, lds, list_n);
-                else tile_block_job<true, false, false>(J, wg, slice, lds, list_n);
-            }
-        } else {
-            if (phase) {
-                if (lut) tile_block_job<false, true, true>(J, wg, slice, lds, list_n);
-                else tile_block_job<false, true, false>(J, wg, slice, lds, list_n);
-            } else {
-                if (lut) tile_block_job<false, false, true>(J, wg, slice, lds, list_n);
-                else tile_block_job<false, false, false>(J, wg, slice, lds, list_n);
-            }
-        }
+        const int wg = blk % per_slice, slice = blk / per_slice;
+        tile_block<ARC, PHASE, LUT, KIND == 2, KIND == 1>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg,
+                                                          slice, lds, list_n);
         __syncthreads();  // the next item reuses the LDS
     }
 }
 
-hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds,
-                                int workgroups, hipStream_t stream) {
-    if (n_items == 0) return hipSuccess;
-    hipError_t e = hipFuncSetAttribute((const void*)project_batch_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+template <bool ARC, bool PHASE, bool LUT>
+static hipError_t launch_batch_kind(int kind, const ProjJob* jobs, const int2* items, int n_items, int* counter,
+                                    size_t lds, int workgroups, hipStream_t stream) {
+    const void* k = kind == 2   ? (const void*)project_batch_kernel<ARC, PHASE, LUT, 2>
+                    : kind == 1 ? (const void*)project_batch_kernel<ARC, PHASE, LUT, 1>
+                                : (const void*)project_batch_kernel<ARC, PHASE, LUT, 0>;
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(project_batch_kernel, dim3(workgroups), dim3(1024), lds, stream, jobs, items, n_items,
-                       counter);
+    if (kind == 2)
+        hipLaunchKernelGGL((project_batch_kernel<ARC, PHASE, LUT, 2>), dim3(workgroups), dim3(1024), lds, stream, jobs,
+                           items, n_items, counter);
+    else if (kind == 1)
+        hipLaunchKernelGGL((project_batch_kernel<ARC, PHASE, LUT, 1>), dim3(workgroups), dim3(1024), lds, stream, jobs,
+                           items, n_items, counter);
+    else
+        hipLaunchKernelGGL((project_batch_kernel<ARC, PHASE, LUT, 0>), dim3(workgroups), dim3(1024), lds, stream, jobs,
+                           items, n_items, counter);
     return hipGetLastError();
+}
+
+hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* items, int n_items, int* counter,
+                                size_t lds, int workgroups, hipStream_t stream) {
+    if (n_items == 0) return hipSuccess;
+    const bool arc = variant & kJobArc, phase = variant & kJobPhase, lut = variant & kJobLut;
+    const int kind = (variant & kJobSpline) ? 2 : (arc && (variant & kJobShared)) ? 1 : 0;
+#define BCSIM_BATCH(A, P, U) return launch_batch_kind<A, P, U>(kind, jobs, items, n_items, counter, lds, workgroups, stream)
+    if (arc) {
+        if (phase) {
+            if (lut) BCSIM_BATCH(true, true, true);
+            BCSIM_BATCH(true, true, false);
+        }
+        if (lut) BCSIM_BATCH(true, false, true);
+        BCSIM_BATCH(true, false, false);
+    }
+    if (phase) {
+        if (lut) BCSIM_BATCH(false, true, true);
+        BCSIM_BATCH(false, true, false);
+    }
+    if (lut) BCSIM_BATCH(false, false, true);
+    BCSIM_BATCH(false, false, false);
+#undef BCSIM_BATCH
 }
 
 size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap) {

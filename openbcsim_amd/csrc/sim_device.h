@@ -259,11 +259,20 @@ hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream);
 // launch_order_tiles for several jobs at once: one workgroup per segment.
 hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t stream);
 
-// Batched frames (one launch each for all jobs of a batch): persistent
-// workgroups take items from `counter` (reset by launch_fill_segments, which
-// also adds one to each of the n_noise per-job noise frame counters).
-hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds,
-                                int workgroups, hipStream_t stream);
+// The JobFlags that select projection code: arc, phase, LUT, spline, and a
+// shared origin (which only matters for fixed points with arc projection).
+inline int batch_variant(int flags) {
+    int f = flags & (kJobArc | kJobPhase | kJobLut | kJobSpline | kJobShared);
+    if (!(f & kJobArc) || (f & kJobSpline)) f &= ~kJobShared;
+    return f;
+}
+
+// Batched frames: persistent workgroups take items from `counter` (reset by
+// launch_fill_segments, which also adds one to each of the n_noise per-job noise
+// frame counters).  launch_project_batch runs the items of one kernel variant
+// (batch_variant of every item's job).
+hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* items, int n_items, int* counter,
+                                size_t lds, int workgroups, hipStream_t stream);
 hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int n_items, int* counter,
                                  int workgroups, hipStream_t stream);
 hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, int n_counters,

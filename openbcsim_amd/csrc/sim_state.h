@@ -1396,9 +1396,29 @@ struct bcsim_sim {
         render_target(t);  // trials project the points of this frame's time
         std::string trials;
         const bool spl = std::strcmp(t.kind, "spline") == 0;
+        // each candidate runs as a frame would: for launches of many blocks, its
+        // blocks heaviest first by the counts of its own previous launch, so two warm
+        // launches settle the order before three are timed
         auto cost = [&](const TileTune& tt) {
-            const TileLaunch cfg = tile_launch(t.nchunks, tt, spl);
-            const double ms = time_launches(stream, [&] { launch_target(t, cfg, bins); }, 2);
+            TileLaunch cfg = tile_launch(t.nchunks, tt, spl);
+            const int blocks = tile_blocks(cfg);
+            const bool ordered = tile_order && blocks >= kMinOrderedBlocks && blocks <= kMaxOrderedTiles;
+            DevBuf order;
+            if (ordered) {
+                order.ensure(2 * (size_t)blocks * sizeof(int));
+                hip_check(launch_fill_zero(order.as<float>(), 2 * (size_t)blocks, stream), "fill kernel");
+                cfg.work = order.as<int>();
+                cfg.order = order.as<int>() + blocks;
+            }
+            const double ms = time_launches(
+                stream,
+                [&] {
+                    if (ordered)
+                        hip_check(launch_order_tiles(cfg.work, const_cast<int*>(cfg.order), blocks, stream),
+                                  "tile order kernel");
+                    launch_target(t, cfg, bins);
+                },
+                3, 2);
             char b[48];
             std::snprintf(b, sizeof b, " [%d,%d,%d,%d]=%.3f", cfg.tile_lines, cfg.block, cfg.slices, cfg.windows, ms);
             trials += b;
@@ -1467,7 +1487,8 @@ struct bcsim_sim {
         double best_ms = 0.0;
         for (int cand : {4, 8}) {
             p.radix = cand;
-            const double ms = time_launches(stream, [&] { hip_check(launch_convolve_demod(p, stream), "convolve"); }, 2);
+            const double ms =
+                time_launches(stream, [&] { hip_check(launch_convolve_demod(p, stream), "convolve"); }, 3, 2);
             char b[32];
             std::snprintf(b, sizeof b, " r%d=%.3f", cand, ms);
             trials += b;

@@ -2,7 +2,7 @@
 algorithm (CpuAlgorithm, src/core/algorithm/CpuAlgorithm.cpp), compiled from the
 reference sources by ``make -C oracle ref`` into oracle/_ref/libbcsim_ref.so.
 
-Run where the reference tree exists:  python tests/golden/make_golden.py
+Run where the reference tree exists:  python tests/golden/make_golden.py [case ...]
 
 Each case is a reduced, seeded workload from openbcsim_amd.workloads; the
 fixture stores the reference's IQ frame ([line][sample] complex64), its
@@ -50,7 +50,29 @@ def golden_cases():
     mix.splines = spl.splines
     mix.name = "mixed_linear_phase"
     cases["mixed_linear_phase"] = mix
+    cases.update(full_size_cases())
     return cases
+
+
+def _lines(wl, idx):
+    o, d, l, t, length = wl.scan
+    idx = np.asarray(idx)
+    wl.scan = (o[idx], d[idx], l[idx], t[idx], length)
+    return wl
+
+
+def full_size_cases():
+    """The benchmark configurations at their real sizes (phantom, line length, FFT
+    length, decimation), restricted to four lines of their scans so the fixtures
+    stay small: the 2D sector (100k points, 0.12 m, N_fft 8192, D=1) with phase
+    delay off and on, the spline M-mode (200k cubic splines, 0.12 m, D=4) and
+    the volume (1M points, 256x64x64 LUT, 0.15 m, N_fft 16384, D=2)."""
+    return {
+        "full_sector": _lines(W.fixed_sector_2d(), [10, 47, 64, 101]),
+        "full_sector_phase": _lines(W.fixed_sector_2d(phase_delay=True), [5, 40, 63, 120]),
+        "full_spline_mmode": _lines(W.spline_mmode(), [3, 77, 128, 250]),
+        "full_volume": _lines(W.volume_lut(), [0, 1111, 2080, 4095]),
+    }
 
 
 def input_digest(wl) -> str:
@@ -71,7 +93,10 @@ def main():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from backends import Reference  # noqa: E402
 
+    only = set(sys.argv[1:])
     for name, wl in golden_cases().items():
+        if only and name not in only:
+            continue
         ref = Reference()
         wl.apply(ref)
         iq = ref.simulate_frame()

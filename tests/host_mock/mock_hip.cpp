@@ -625,8 +625,8 @@ hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t s
     return hipSuccess;
 }
 
-hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds, int,
-                                hipStream_t s) {
+hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* items, int n_items, int* counter,
+                                size_t lds, int, hipStream_t s) {
     if (n_items == 0) return hipSuccess;
     enqueue(s, Op{[=] {
                       const char* op = "project_batch";
@@ -635,7 +635,11 @@ hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_it
                       int nj = 0;
                       for (int i = 0; i < n_items; i++) nj = std::max(nj, items[i].x + 1);
                       check(jobs, (size_t)nj * sizeof(ProjJob), "jobs", op);
-                      for (int j = 0; j < nj; j++) {
+                      for (int i = 0; i < n_items; i++)
+                          if (batch_variant(jobs[items[i].x].flags) != variant) die("project_batch: item of another variant");
+                      std::set<int> present;  // the jobs of this launch's variant
+                      for (int i = 0; i < n_items; i++) present.insert(items[i].x);
+                      for (int j : present) {
                           const ProjJob& J = jobs[j];
                           const bool spl = J.flags & kJobSpline;
                           check_tile(J.c, spl ? nullptr : &J.fs, spl ? &J.ss : nullptr, J.boxes, J.nchunks, J.bins,
@@ -652,7 +656,7 @@ hipError_t launch_project_batch(const ProjJob* jobs, const int2* items, int n_it
                           if (!seen.insert({items[i].x, items[i].y}).second) die("project_batch: item listed twice");
                       }
                       size_t all = 0;
-                      for (int j = 0; j < nj; j++) all += (size_t)jobs[j].blocks_per_slice * jobs[j].g.slices;
+                      for (int j : present) all += (size_t)jobs[j].blocks_per_slice * jobs[j].g.slices;
                       if (all != (size_t)n_items) die("project_batch: items do not cover every block");
                       if (lds + 8 > kLds) die("project_batch: LDS");
                   },

@@ -96,3 +96,36 @@ def test_batch_config3_jobs_match_single():
         worst = max(worst, e)
     print(f"[result] 64-job batch worst rel_l2 vs single = {worst:.2e}", flush=True)
     assert worst < 1e-6
+
+
+def test_full_scale_batch_jobs_against_oracle():
+    """The 64 uneven jobs of the batched workload at full scale (10k-2M
+    scatterers, fixed and spline, analytic and LUT, FFT 2048-16384): eight of
+    them, from the smallest to the largest, taken from one batched frame and
+    compared on four lines each with the FP64 oracle's exact sum."""
+    from openbcsim_amd.batch import Batch
+    jobs = W.batched_jobs(64, seed=3)
+    sims = [make_sim(wl) for wl in jobs]
+    b = Batch(sims)
+    frames = b.simulate()
+    order = sorted(range(len(jobs)), key=lambda j: jobs[j].num_scatterers)
+    picks = [order[k] for k in (0, 9, 18, 27, 36, 45, 54, 63)]
+    kinds = set()
+    worst = 0.0
+    for j in picks:
+        wl = jobs[j]
+        kinds.add(("spline" if wl.splines else "fixed", wl.profile[0]))
+        o, d, l, t, length = wl.scan
+        sel = np.unique(np.linspace(0, o.shape[0] - 1, 4).astype(int))
+        wl.scan = (o[sel], d[sel], l[sel], t[sel], length)
+        orc = Oracle()
+        wl.apply(orc)
+        e = rel_l2(frames[j][sel], orc.simulate_frame())
+        print(f"[result] job {j}: {wl.num_scatterers} scatterers, N_fft {wl.fft_len()}, "
+              f"{'spline' if wl.splines else 'fixed'}/{wl.profile[0]}: rel_l2 vs oracle {e:.2e}", flush=True)
+        worst = max(worst, e)
+    assert worst <= 1e-4
+    assert len(kinds) >= 3, kinds  # the picks span the job kinds
+    b.close()
+    for s in sims:
+        s.close()
