@@ -110,10 +110,13 @@ def run_gpu(wl, steps: int, warmup: int, tile=None):
                 kern.setdefault(key, []).extend(sim.get_debug_data(key))
             except RuntimeError:
                 pass
-        # the projection launch's own counts (the same every frame)
-        for key in ("projection_weight_evals", "projection_lds_adds"):
-            counts[key] = float(sum(sim.get_debug_data(key)))
     sim.set_parameter("store_kernel_details", "off")
+    # the projection launch's own counts (the same every frame), from one more frame
+    sim.set_parameter("projection_counters", "on")
+    sim.simulate_lines_device(out.address, L * S)
+    for key in ("projection_weight_evals", "projection_lds_adds", "projection_chunk_line_pairs"):
+        counts[key] = float(sum(sim.get_debug_data(key)))
+    sim.set_parameter("projection_counters", "off")
     # PCIe-inclusive rate (host output), reported beside the device-resident value
     sim.simulate_frame()
     t2 = time.perf_counter()

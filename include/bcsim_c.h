@@ -70,7 +70,16 @@ BCSIM_API bcsim_status bcsim_destroy(bcsim_sim* sim);
  * (on/off, default on: large projection launches run their blocks heaviest
  * first, by the previous frame's counts); chunk_order (on/off, default on: the
  * projection visits the chunks near to far from the scan origin).  Neither
- * changes a frame beyond float summation order.
+ * changes a frame beyond float summation order.  weight_cutoff (default 1e-8,
+ * in [0, 1)): beam-profile weights below this fraction of the profile's peak
+ * magnitude (1 for the Gaussian, the largest |sample| of a LUT) count as zero
+ * and their pairs are culled; each dropped pair changes its bin by less than
+ * the cutoff times a full-weight echo of the same amplitude; 0 sums every pair
+ * whose weight is not exactly zero, as the reference does.  projection_counters
+ * (on/off, default off): every projection launch counts its profile-weight
+ * evaluations, LDS bin additions and culled-in (chunk, line) pairs into the
+ * debug keys projection_weight_evals, projection_lds_adds and
+ * projection_chunk_line_pairs (one value per launch).
  * Read-only keys of bcsim_get_parameter: launch_plan (shapes of the last frame)
  * and autotune_report (every tuned class with its trial timings). */
 BCSIM_API bcsim_status bcsim_set_parameter(bcsim_sim* sim, const char* key, const char* value);
@@ -161,9 +170,10 @@ BCSIM_API bcsim_status bcsim_synchronize(bcsim_sim* sim);
  * configuration: after configuring it (or changing its configuration) run one
  * frame through bcsim_simulate_lines_device or bcsim_simulate_lines, else
  * BCSIM_ERR_INVALID_STATE.  Launch shapes are the user-set or heuristic ones (no
- * tuning trials), blocks run in index order, and store_kernel_details records
- * nothing; the IQ equals bcsim_simulate_lines_device's up to float summation
- * order.  bcsim_debug_get_projection fails after such a frame. */
+ * tuning trials), blocks run in index order, and store_kernel_details and
+ * projection_counters record nothing; the IQ equals that of
+ * bcsim_simulate_lines_device up to float summation order.
+ * bcsim_debug_get_projection fails after such a frame. */
 BCSIM_API bcsim_status bcsim_simulate_lines_device_noalloc(bcsim_sim* sim, void* d_out, size_t capacity,
                                                            int* num_lines, int* num_samples);
 /* The handle's hipStream_t, as void*. */
@@ -171,9 +181,13 @@ BCSIM_API bcsim_status bcsim_get_stream(bcsim_sim* sim, void** stream);
 
 /* IAlgorithm::get_debug_data (LibBCSim.hpp:84; BaseAlgorithm.cpp:89-94).
  * With store_kernel_details on, keys hold per-launch milliseconds measured with
- * HIP events on the handle's stream: kernel_memset_ms, fixed_projection_kernel_ms,
- * spline_projection_kernel_ms, kernel_render_ms, kernel_convolve_demodulate_ms,
- * kernel_memcpy_ms.  *count receives the number of values; at most capacity are
+ * HIP events on the handle's stream: kernel_memset_ms (0 when no fill ran),
+ * fixed_projection_kernel_ms, spline_projection_kernel_ms, kernel_render_ms,
+ * kernel_convolve_demodulate_ms, kernel_memcpy_ms (0 for device output); and,
+ * as the reference records them (GpuAlgorithm.cpp:193-358), stream_numbers (0
+ * for every line: one stream per frame) and kernel_forward_fft_ms (the fused
+ * convolution kernel's time) with kernel_multiply_fft_ms, kernel_inverse_fft_ms
+ * and kernel_demodulate_ms 0 (fused into that kernel).  *count receives the number of values; at most capacity are
  * written.  Unknown keys fail like the reference ("invalid debug data key"). */
 BCSIM_API bcsim_status bcsim_get_debug_data(bcsim_sim* sim, const char* key, double* out,
                                             size_t capacity, size_t* count);

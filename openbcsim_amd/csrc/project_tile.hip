@@ -201,6 +201,8 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     }
     zero_lds(bins, nl * stride);
     if (threadIdx.x == 0) list_n = 0;
+    __shared__ unsigned long long wg_stats[3];  // the block's launch statistics (TileLaunch::stats)
+    if (g.stats && threadIdx.x < 3) wg_stats[threadIdx.x] = 0;
 
     // radial extent of this window, intersected with the profile's radial support
     const float r_lo = fmaxf(cp.r_lo, ((float)b0 - 0.5f) * cp.r_per_bin);
@@ -411,17 +413,20 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
         if (lane == 0 && hits) atomicAdd(&g.work[wg], hits);
     }
     if (g.stats) {
-        unsigned long long ev = (unsigned)tally.evals, ad = (unsigned)tally.adds;
+        unsigned long long ev = (unsigned)tally.evals, ad = (unsigned)tally.adds, hc = (unsigned)hits;
         for (int off = 32; off > 0; off >>= 1) {
             ev += __shfl_xor(ev, off);
             ad += __shfl_xor(ad, off);
+            if (!g.work) hc += __shfl_xor(hc, off);  // (already summed over the wave for the block order)
         }
         if (lane == 0) {
-            atomicAdd(&g.stats[0], ev);
-            atomicAdd(&g.stats[1], ad);
+            atomicAdd(&wg_stats[0], ev);
+            atomicAdd(&wg_stats[1], ad);
+            atomicAdd(&wg_stats[2], hc);
         }
     }
     __syncthreads();
+    if (g.stats && threadIdx.x < 3) atomicAdd(&g.stats[threadIdx.x], wg_stats[threadIdx.x]);  // one per block
     // flush: line k's window [b0, b0 + nb) of the global bins [line][num_rf]
     const int row = nb * E;
     for (int i = threadIdx.x; i < nl * row; i += blockDim.x) {
@@ -562,7 +567,7 @@ static hipError_t launch_batch_kind(int kind, const ProjJob* jobs, const int2* i
     const void* k = kind == 2   ? (const void*)project_batch_kernel<ARC, PHASE, LUT, 2>
                     : kind == 1 ? (const void*)project_batch_kernel<ARC, PHASE, LUT, 1>
                                 : (const void*)project_batch_kernel<ARC, PHASE, LUT, 0>;
-    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     if (kind == 2)
         hipLaunchKernelGGL((project_batch_kernel<ARC, PHASE, LUT, 2>), dim3(workgroups), dim3(1024), lds, stream, jobs,
@@ -610,7 +615,7 @@ static hipError_t launch_tile_t(const ProjCommon& c, const CullParams& cp, const
     const int tiles = (c.num_lines + cfg.tile_lines - 1) / cfg.tile_lines;
     const size_t lds = tile_lds_bytes(cfg.window_bins, PHASE, cfg.tile_lines, cfg.list_cap);
     auto k = project_tile_kernel<ARC, PHASE, LUT, SPLINE, SHARED>;
-    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = allow_lds((const void*)k, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(tiles * cfg.windows, cfg.slices), dim3(cfg.block), lds, stream, c, cp, fs, ss, boxes,
                        nchunks, bins, cfg);

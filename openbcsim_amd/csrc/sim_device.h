@@ -184,7 +184,8 @@ struct TileLaunch {
     int* work;                // per-block hit counts added up by the launch, or null
     const int* chunks;        // chunk visiting order (a permutation of [0, nchunks)), or null: index order
     // or null: per launch, [0] profile-weight evaluations (LUT record gathers /
-    // Gaussian exponentials) and [1] LDS bin additions, added up by the launch
+    // Gaussian exponentials), [1] LDS bin additions and [2] (chunk, line) pairs
+    // the cull kept (each evaluated by a wave of 64 lanes), added up by the launch
     unsigned long long* stats;
 };
 
@@ -232,6 +233,11 @@ constexpr int kMaxScratchDevices = 64;
 // Device address of the scratch array on `device` (the current device), looked
 // up once per device with hipGetSymbolAddress and kept.
 hipError_t noalloc_scratch(int device, void** ptr);
+
+// Let kernel `func` use up to `bytes` of dynamic LDS: the attribute is raised to
+// the largest dynamic LDS any launch of this library asks for, once per kernel
+// and device (sim_kernels.hip), so launches do not pay for the call.
+hipError_t allow_lds(const void* func, size_t bytes);
 
 // Launchers (sim_kernels.hip).  All enqueue on `stream` and return the launch status.
 hipError_t launch_fill_zero(float* p, size_t n_floats, hipStream_t stream);

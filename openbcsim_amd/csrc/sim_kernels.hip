@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <set>
+#include <utility>
 
 #include "proj_math.h"
 #include "sim_device.h"
@@ -56,6 +58,21 @@ hipError_t noalloc_scratch(int device, void** ptr) {
 
 // ------------------------------------------------------------------------------------
 // utilities
+
+hipError_t allow_lds(const void* func, size_t bytes) {
+    constexpr size_t kMaxDynamic = (size_t)kLdsBytes - 256;  // below the static LDS of every kernel here
+    if (bytes > kMaxDynamic) return hipErrorInvalidValue;
+    static std::mutex mu;
+    static std::set<std::pair<int, const void*>> raised;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(mu);
+    if (raised.count({dev, func})) return hipSuccess;
+    e = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxDynamic);
+    if (e == hipSuccess) raised.insert({dev, func});
+    return e;
+}
 
 __global__ void fill_zero_kernel(float4* p4, size_t n4, float* tail, int n_tail) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -151,7 +168,7 @@ hipError_t launch_project_fixed(const ProjCommon& c, const FixedSet& s, void* bi
     const int chunk = (s.n + cfg.slices - 1) / cfg.slices;
     const size_t lds = proj_lds_bytes(c, complex_bins, cfg.tile_lines);
     auto k = BCSIM_SELECT(project_fixed_kernel, arc, phase, lut);
-    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = allow_lds((const void*)k, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(tiles, cfg.slices), dim3(cfg.block), lds, stream, c, s,
                        static_cast<float*>(bins), cfg.tile_lines, chunk, cfg.atomic);
@@ -166,7 +183,7 @@ hipError_t launch_project_spline(const ProjCommon& c, const SplineSet& s, void* 
     const int chunk = (s.n + cfg.slices - 1) / cfg.slices;
     const size_t lds = proj_lds_bytes(c, complex_bins, cfg.tile_lines);
     auto k = BCSIM_SELECT(project_spline_kernel, arc, phase, lut);
-    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = allow_lds((const void*)k, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(tiles, cfg.slices), dim3(cfg.block), lds, stream, c, s,
                        static_cast<float*>(bins), cfg.tile_lines, chunk, cfg.atomic);
@@ -542,8 +559,7 @@ hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int 
                                  int workgroups, hipStream_t stream) {
     if (n_items == 0) return hipSuccess;
     const size_t lds = convolve_batch_lds_bytes();
-    hipError_t e = hipFuncSetAttribute((const void*)convolve_batch_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = allow_lds((const void*)convolve_batch_kernel, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(convolve_batch_kernel, dim3(workgroups), dim3(1024), lds, stream, jobs, items, n_items,
                        counter);
@@ -578,7 +594,7 @@ int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
 template <int NT, int NMAX, bool CB, int R>
 static hipError_t launch_conv(const ConvParams& p, size_t lds, hipStream_t stream) {
     const void* k = (const void*)convolve_demod_kernel<NT, NMAX, CB, R>;
-    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((convolve_demod_kernel<NT, NMAX, CB, R>), dim3(p.num_lines), dim3(NT), lds, stream, p);
     return hipGetLastError();

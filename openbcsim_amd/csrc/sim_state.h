@@ -303,6 +303,7 @@ struct bcsim_sim {
     int num_streams = 2;
     int threads_per_block = 128;
     bool store_details = false;
+    bool count_pairs = false;  // projection_counters: the launches count their work (debug data)
     // Beam-profile weights below this fraction of the profile's peak magnitude
     // (1 for the Gaussian, the largest |sample| of a LUT) count as zero, so their
     // pairs are culled; 0 keeps every pair whose weight is not exactly zero.
@@ -437,6 +438,8 @@ struct bcsim_sim {
             threads_per_block = n;  // launch shapes come from the tuner; kept for compatibility
         } else if (key == "store_kernel_details") {
             store_details = parse_bool(value);
+        } else if (key == "projection_counters") {
+            count_pairs = parse_bool(value);
         } else if (key == "weight_cutoff") {
             const float v = parse_float(value);
             if (!(v >= 0.0f && v < 1.0f)) fail(BCSIM_ERR_INVALID_ARGUMENT, "weight_cutoff must lie in [0, 1)");
@@ -492,6 +495,7 @@ struct bcsim_sim {
         if (key == "cuda_streams") return std::to_string(num_streams);
         if (key == "threads_per_block") return std::to_string(threads_per_block);
         if (key == "store_kernel_details") return store_details ? "on" : "off";
+        if (key == "projection_counters") return count_pairs ? "on" : "off";
         if (key == "projection_mode") return direct_mode ? "direct" : "tiled";
         if (key == "weight_cutoff") {
             char b[32];
@@ -1108,19 +1112,21 @@ struct bcsim_sim {
     std::vector<Pending> pending;
 
     size_t stats_launches = 0;  // projection launches of the last frame that filled d_stats
+    static constexpr size_t kStatsPerLaunch = 3;  // TileLaunch::stats
 
     void collect_timings() {
         if (stats_launches) {
             // the launches' own counts (TileLaunch::stats): the weight evaluations and
             // LDS additions of each projection launch
-            std::vector<unsigned long long> st(2 * stats_launches);
+            std::vector<unsigned long long> st(kStatsPerLaunch * stats_launches);
             hip_check(hipMemcpyAsync(st.data(), d_stats.p, st.size() * sizeof(unsigned long long),
                                      hipMemcpyDeviceToHost, stream),
                       "hipMemcpyAsync");
             hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
             for (size_t i = 0; i < stats_launches; i++) {
-                debug["projection_weight_evals"].push_back((double)st[2 * i]);
-                debug["projection_lds_adds"].push_back((double)st[2 * i + 1]);
+                debug["projection_weight_evals"].push_back((double)st[kStatsPerLaunch * i]);
+                debug["projection_lds_adds"].push_back((double)st[kStatsPerLaunch * i + 1]);
+                debug["projection_chunk_line_pairs"].push_back((double)st[kStatsPerLaunch * i + 2]);
             }
             stats_launches = 0;
         }
@@ -1515,10 +1521,8 @@ struct bcsim_sim {
     // trials, no block-order buffers, no timing events).
     void enqueue_frame(float2* d_out, FrameMem* fm = nullptr) {
         if (!fm) prepare();
-        if (store_details) {
-            debug.clear();
-            record_fused_keys();
-        }
+        if (store_details || count_pairs) debug.clear();
+        if (store_details) record_fused_keys();
         const bool cplx = phase_delay;
         const size_t L = lines.size();
         const size_t bin_floats = L * (size_t)num_rf * (cplx ? 2 : 1);
@@ -1556,16 +1560,17 @@ struct bcsim_sim {
             debug["kernel_memset_ms"].push_back(0.0);  // every bin is stored by the projection
         }
         stats_launches = 0;
-        if (store_details && !direct_mode && !targets.empty()) {
-            d_stats.ensure(2 * sizeof(unsigned long long) * targets.size());
-            hip_check(launch_fill_zero(d_stats.as<float>(), 4 * targets.size(), stream), "fill kernel");
+        if (count_pairs && !direct_mode && !targets.empty()) {
+            d_stats.ensure(kStatsPerLaunch * sizeof(unsigned long long) * targets.size());
+            hip_check(launch_fill_zero(d_stats.as<float>(), 2 * kStatsPerLaunch * targets.size(), stream), "fill kernel");
             stats_launches = targets.size();
         }
         for (size_t i = 0; i < targets.size(); i++) {
             const ProjTarget& t = targets[i];
             const bool is_spline = std::strcmp(t.kind, "fixed") != 0;
-            Timed tm(this, is_spline ? "spline_projection_kernel_ms" : "fixed_projection_kernel_ms");
+            const char* tkey = is_spline ? "spline_projection_kernel_ms" : "fixed_projection_kernel_ms";
             if (direct_mode) {
+                Timed tm(this, tkey);
                 ProjLaunch cfg = direct_plans[i];
                 cfg.atomic = (cfg.slices > 1 || !single) ? 1 : 0;
                 if (std::strcmp(t.kind, "spline") == 0)
@@ -1574,10 +1579,11 @@ struct bcsim_sim {
                 else
                     hip_check(launch_project_fixed(t.c, t.fs, bins, cplx, arc, phase_delay, lut_profile, cfg, stream),
                               is_spline ? "spline projection kernel" : "fixed projection kernel");
+                tm.stop();
             } else {
                 TileLaunch cfg = plans[i];
                 cfg.atomic |= single ? 0 : 1;
-                if (stats_launches) cfg.stats = d_stats.as<unsigned long long>() + 2 * i;
+                if (stats_launches) cfg.stats = d_stats.as<unsigned long long>() + kStatsPerLaunch * i;
                 const int blocks = tile_blocks(cfg);
                 if (!fm && tile_order && blocks >= kMinOrderedBlocks && blocks <= kMaxOrderedTiles) {
                     int* wo = order_buffer(i, blocks);
@@ -1586,9 +1592,10 @@ struct bcsim_sim {
                     hip_check(launch_order_tiles(cfg.work, const_cast<int*>(cfg.order), blocks, stream),
                               "tile order kernel");
                 }
+                Timed tm(this, tkey);  // the projection kernel alone
                 launch_target(t, cfg, bins);
+                tm.stop();
             }
-            tm.stop();
         }
         ConvParams p = conv_params(d_out);
         p.bins = bins;
@@ -1649,10 +1656,8 @@ struct bcsim_sim {
         if (ns) *ns = n_out;
         last_frame_noalloc = false;
         enqueue_frame(static_cast<float2*>(d_out));
-        if (store_details) {
-            debug["kernel_memcpy_ms"].push_back(0.0);  // the IQ stays in device memory
-            collect_timings();
-        }
+        if (store_details) debug["kernel_memcpy_ms"].push_back(0.0);  // the IQ stays in device memory
+        if (store_details || stats_launches) collect_timings();
     }
 
     bool last_frame_noalloc = false;  // the last frame's bins were in the shared scratch
@@ -1662,7 +1667,7 @@ struct bcsim_sim {
     // allocation-free frame on the GPU shares.  Everything else a frame reads must
     // already exist: the simulator's tables are built by its first frame through
     // simulate_device or simulate_host, and a configuration change needs such a
-    // frame again.  No launch-shape tuning, block ordering or kernel timing.
+    // frame again.  No launch-shape tuning, block ordering, kernel timing or counters.
     void simulate_device_noalloc(void* d_out, size_t capacity, int* nl, int* ns) {
         require_configured();
         if (lines.empty()) fail(BCSIM_ERR_INVALID_STATE, "No scanlines in scansequence");
@@ -1691,15 +1696,17 @@ struct bcsim_sim {
         fm.scratch = static_cast<unsigned char*>(base);
         fm.cap = kNoAllocScratchBytes;
         fm.bins = static_cast<float*>(fm.carve(lines.size() * (size_t)num_rf * (phase_delay ? 8 : 4)));
-        const bool details = store_details;
-        store_details = false;  // timing events are created per frame
+        const bool details = store_details, counters = count_pairs;
+        store_details = count_pairs = false;  // timing events and counters need allocations of their own
         try {
             enqueue_frame(static_cast<float2*>(d_out), &fm);
         } catch (...) {
             store_details = details;
+            count_pairs = counters;
             throw;
         }
         store_details = details;
+        count_pairs = counters;
         last_frame_noalloc = true;
         if (nl) *nl = (int)lines.size();
         if (ns) *ns = n_out;
@@ -1723,7 +1730,7 @@ struct bcsim_sim {
             }
             hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         }
-        if (store_details) collect_timings();
+        if (store_details || stats_launches) collect_timings();
         if (nl) *nl = (int)lines.size();
         if (ns) *ns = n_out;
     }

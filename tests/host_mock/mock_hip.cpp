@@ -579,7 +579,7 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams&, const Fix
     enqueue(s, Op{[=] {
                       check_tile(c, is_spline ? nullptr : &fs, is_spline ? &ss : nullptr, boxes, nchunks,
                                  (const float*)bins, phase, lut, cfg, cfg.block, "project_tile");
-                      if (cfg.stats) check(cfg.stats, 2 * sizeof(unsigned long long), "stats", "project_tile");
+                      if (cfg.stats) check(cfg.stats, 3 * sizeof(unsigned long long), "stats", "project_tile");
                   },
                   "project_tile"});
     return hipSuccess;

@@ -114,10 +114,12 @@ def test_launch_counters():
         g = RfSimulator("gpu")
         wl.apply(g)
         g.set_parameter("weight_cutoff", str(cutoff))
-        g.set_parameter("store_kernel_details", "on")
+        g.set_parameter("projection_counters", "on")
         g.simulate_frame()
         ev = g.get_debug_data("projection_weight_evals")
         ad = g.get_debug_data("projection_lds_adds")
+        pairs = g.get_debug_data("projection_chunk_line_pairs")
+        assert len(pairs) == 1 and 64 * pairs[0] >= ev[0]  # each kept (chunk, line) pair is 64 lanes
         g.close()
         assert len(ev) == 1 and len(ad) == 1
         counts[cutoff] = (ev[0], ad[0])
