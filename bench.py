@@ -78,12 +78,14 @@ def projection_key(wl) -> str:
     return "fixed_projection_kernel_ms" if wl.fixed else "spline_projection_kernel_ms"
 
 
-def run_gpu(wl, steps: int, warmup: int, tile=None):
+def run_gpu(wl, steps: int, warmup: int, tile=None, bins=None):
     from openbcsim_amd.hip_rt import DeviceBuffer, hip
     from openbcsim_amd.pyrfsim import RfSimulator
 
     sim = RfSimulator("gpu")
     wl.apply(sim)
+    if bins:
+        sim.set_parameter("bin_accumulation", bins)
     if tile:
         for key, val in zip(("tile_windows", "tile_lines", "tile_block"), tile.split(",")):
             sim.set_parameter(key, val)
@@ -117,6 +119,7 @@ def run_gpu(wl, steps: int, warmup: int, tile=None):
     for key in ("projection_weight_evals", "projection_lds_adds", "projection_chunk_line_pairs"):
         counts[key] = float(sum(sim.get_debug_data(key)))
     sim.set_parameter("projection_counters", "off")
+    counts["bins"] = "fixed" if "bins=fixed" in sim.get_parameter("launch_plan") else "float"
     # PCIe-inclusive rate (host output), reported beside the device-resident value
     sim.simulate_frame()
     t2 = time.perf_counter()
@@ -133,8 +136,9 @@ def run_configs(steps: int = 5):
     """Every single-simulator config with phase delay off and on (device-resident IQ,
     frame time over `steps` frames after one warm-up): the 2D sector (100k points,
     128 lines), the spline M-mode (200k cubic splines, 256 lines, one time per
-    line) and the 1M-point volume; each again with weight_cutoff 0 (every pair
-    whose weight is not exactly zero, as the reference sums them)."""
+    line) and the 1M-point volume, with the library's choice of LDS bins; each
+    again with fixed-point and with float LDS bins forced, and with weight_cutoff 0
+    (every pair whose weight is not exactly zero, as the reference sums them)."""
     from openbcsim_amd import workloads as W
     from openbcsim_amd.hip_rt import DeviceBuffer
     from openbcsim_amd.pyrfsim import RfSimulator
@@ -148,24 +152,29 @@ def run_configs(steps: int = 5):
             L, S = sim.output_shape()
             buf = DeviceBuffer(L * S * 8)
             res = {}
-            for cutoff in (None, "0"):
-                if cutoff is not None:
-                    sim.set_parameter("weight_cutoff", cutoff)
+            for variant, params in (("default", {}), ("fixed_bins", {"bin_accumulation": "fixed"}),
+                                    ("float_bins", {"bin_accumulation": "float"}),
+                                    ("cutoff_off", {"bin_accumulation": "auto", "weight_cutoff": "0"})):
+                for key, val in params.items():
+                    sim.set_parameter(key, val)
                 sim.simulate_lines_device(buf.address, L * S)
                 sim.synchronize()
                 t0 = time.perf_counter()
                 for _ in range(steps):
                     sim.simulate_lines_device(buf.address, L * S)
                 sim.synchronize()
-                res[cutoff] = (time.perf_counter() - t0) / steps
-            dt = res[None]
+                res[variant] = (time.perf_counter() - t0) / steps
+            dt = res["default"]
             out[f"{name}{'_phase' if phase else ''}"] = {
                 "lines": L, "scatterers": wl.num_scatterers, "ms_per_frame": round(dt * 1e3, 4),
                 "iq_lines_per_s": round(L / dt, 1), "scatterer_evals_per_s": round(L * wl.num_scatterers / dt, 1),
-                "ms_per_frame_cutoff_off": round(res["0"] * 1e3, 4),
-                "iq_lines_per_s_cutoff_off": round(L / res["0"], 1)}
+                "ms_per_frame_fixed_bins": round(res["fixed_bins"] * 1e3, 4),
+                "ms_per_frame_float_bins": round(res["float_bins"] * 1e3, 4),
+                "ms_per_frame_cutoff_off": round(res["cutoff_off"] * 1e3, 4),
+                "iq_lines_per_s_cutoff_off": round(L / res["cutoff_off"], 1)}
             log(f"config {name}{'_phase' if phase else ''}: {dt * 1e3:.4f} ms per frame "
-                f"({res['0'] * 1e3:.4f} without the weight cutoff)")
+                f"(fixed bins {res['fixed_bins'] * 1e3:.4f}, float bins {res['float_bins'] * 1e3:.4f}, "
+                f"{res['cutoff_off'] * 1e3:.4f} without the weight cutoff)")
             sim.close()
             buf.close()
     return out
@@ -685,7 +694,7 @@ def measure_traffic(workload: str, keep_dir=None):
     return res, None
 
 
-def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings):
+def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings, fixed_bins=False):
     """The dominant kernel against HBM (measured bytes) and against the measured
     ceilings of the units its counters show busy: LDS float atomics and LUT record
     gathers (the launch's own counts) and VALU issue (SQ_INSTS_VALU)."""
@@ -710,9 +719,12 @@ def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings
     if ceilings:
         adds, evals = (counts or {}).get("projection_lds_adds"), (counts or {}).get("projection_weight_evals")
         if adds:
-            c = ceilings["lds_float_atomics_per_s"]
-            units["lds_float_atomics"] = {"per_launch": adds, "achieved_per_s": adds / t, "ceiling_per_s": c,
-                                          "frac": round(adds / t / c, 4)}
+            # the launch's bins: float (ds_add_f32) or 64-bit fixed point (ds_add_u64)
+            name, key = (("lds_int64_atomics", "lds_int64_atomics_per_s") if fixed_bins else
+                         ("lds_float_atomics", "lds_float_atomics_per_s"))
+            c = ceilings[key]
+            units[name] = {"per_launch": adds, "achieved_per_s": adds / t, "ceiling_per_s": c,
+                           "frac": round(adds / t / c, 4)}
         if evals and wl.profile[0] == "lut":
             c = ceilings["lut_gathers_per_s"]
             units["lut_record_gathers"] = {"per_launch": evals, "achieved_per_s": evals / t, "ceiling_per_s": c,
@@ -727,7 +739,9 @@ def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings
         rl["units"] = units
         rl["ceilings"] = {k: round(v, 1) for k, v in ceilings.items()}
         if units:
-            rl["binding_unit"] = max(units, key=lambda k: units[k]["frac"])
+            # the unit nearest its ceiling (not necessarily the one that bounds the
+            # kernel: the units overlap, and a unit made faster may not move it)
+            rl["busiest_unit"] = max(units, key=lambda k: units[k]["frac"])
     return rl
 
 
@@ -750,6 +764,8 @@ def main():
                     help="length of the timed serving run (the numpy-output comparison runs 10 s at most)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--tile", default=None, help="projection launch shape 'windows,lines,block' (default: automatic)")
+    ap.add_argument("--bins", default=None, choices=("auto", "fixed", "float"),
+                    help="LDS bin accumulation of the projection (default: the library's automatic choice)")
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args.workload)
@@ -768,7 +784,7 @@ def main():
         log(f"traffic: {traffic if traffic else traffic_err}")
     wl = build_workload(args.workload)
     log(f"workload {args.workload}: {wl.num_lines} lines x {wl.num_scatterers} scatterers")
-    L, S, ms, kern, host_ms, counts = run_gpu(wl, args.steps, args.warmup, args.tile)
+    L, S, ms, kern, host_ms, counts = run_gpu(wl, args.steps, args.warmup, args.tile, args.bins)
     log(f"{ms:.4f} ms per frame; kernels {kern}")
     N = wl.num_scatterers
     evals = L * N
@@ -777,7 +793,9 @@ def main():
     from openbcsim_amd import ceilings as CE
     ceil = CE.measure()
     log(f"ceilings: {ceil}")
-    roofline = roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceil) if proj_ms else None
+    lds_bins = counts.pop("bins", None)
+    roofline = (roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceil,
+                                fixed_bins=lds_bins == "fixed") if proj_ms else None)
     line = {
         "metric": "IQ lines/s & scatterer-evals/s at 1M scatterers; serving p50/p99 frame latency",
         "value": round(L / (ms * 1e-3), 1),
@@ -794,7 +812,7 @@ def main():
         "config": {"workload": args.workload, "lines": L, "scatterers": N, "samples_per_line": S,
                    "fft_len": wl.fft_len(), "decimation": int(wl.params.get("radial_decimation", 1)),
                    "profile": wl.profile[0], "phase_delay": wl.params.get("phase_delay", "off"),
-                   "weight_cutoff": 1e-8},
+                   "weight_cutoff": 1e-8, "lds_bins": lds_bins},
         "scatterer_evals_per_s": round(evals / (ms * 1e-3), 1),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "projection_counts": counts,

@@ -79,7 +79,14 @@ BCSIM_API bcsim_status bcsim_destroy(bcsim_sim* sim);
  * (on/off, default off): every projection launch counts its profile-weight
  * evaluations, LDS bin additions and culled-in (chunk, line) pairs into the
  * debug keys projection_weight_evals, projection_lds_adds and
- * projection_chunk_line_pairs (one value per launch).
+ * projection_chunk_line_pairs (one value per launch).  bin_accumulation (auto,
+ * fixed or float; default auto): the projection's LDS bins as 64-bit fixed point
+ * (integer LDS atomics; each contribution rounded to a power-of-two grid chosen
+ * per launch from the dataset's amplitudes, about 1e-12 of a typical one, then
+ * summed exactly) or float; auto tunes the choice where tuning is on and is
+ * otherwise fixed for the Gaussian profile and float for a LUT.  Datasets whose
+ * amplitudes the grid cannot resolve (a non-finite amplitude, or a dynamic
+ * range beyond ~2^39/n) use float bins whatever the setting.
  * Read-only keys of bcsim_get_parameter: launch_plan (shapes of the last frame)
  * and autotune_report (every tuned class with its trial timings). */
 BCSIM_API bcsim_status bcsim_set_parameter(bcsim_sim* sim, const char* key, const char* value);

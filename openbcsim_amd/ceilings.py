@@ -4,7 +4,8 @@ libbcsim_ceilings.so:
 
 * ``lds_add_rate``: ds_add_f32 per second, chip-wide, with the tile kernel's
   pattern (4 rows of 9740 bins per 1024-thread workgroup, each wave adding at
-  bins spread over a 256-bin window);
+  bins spread over a 256-bin window); mode 2: ds_add_u64 into 2 rows of 64-bit
+  bins (the fixed-point bins);
 * ``gather_rate``: 32-byte LUT record gathers per second, each wave's lanes in
   a chunk-sized neighbourhood of an L2-resident table (the projection's LUT
   lookups with nothing else in the way);
@@ -90,6 +91,7 @@ def measure() -> dict:
     """Every ceiling, in the units the benchmark reports."""
     return {"lds_float_atomics_per_s": lds_add_rate(),
             "lds_float_atomics_per_s_conflict_free": lds_add_rate(mode=1),
+            "lds_int64_atomics_per_s": lds_add_rate(mode=2),
             "lut_gathers_per_s": gather_rate(),
             "lut_gathers_per_s_one_line_per_lane": gather_rate(span=(32, 64, 64)),
             "valu_wave_instructions_per_s": valu_rate(),

@@ -18,12 +18,13 @@ namespace bcsim_hip {
 // Tile-kernel launch shape; 0 in a field means "the heuristic's choice".
 struct TileTune {
     int windows = 0, lines = 0, block = 0, slices = 0;
+    int bins = 0;  // LDS bin accumulation: 1 fixed point, 2 float, 0 automatic
 };
 
 inline std::string describe(const TileTune& t) {
-    char buf[96];
-    std::snprintf(buf, sizeof buf, "lines=%d block=%d slices=%d windows=%d", t.lines, t.block, t.slices,
-                  t.windows);
+    char buf[112];
+    std::snprintf(buf, sizeof buf, "lines=%d block=%d slices=%d windows=%d bins=%s", t.lines, t.block, t.slices,
+                  t.windows, t.bins == 1 ? "fixed" : t.bins == 2 ? "float" : "auto");
     return buf;
 }
 

@@ -233,13 +233,14 @@ struct bcsim_batch {
                 if (tile_lines > 0) {
                     TileTune tt = s->tune;
                     tt.lines = tile_lines;
-                    J.g = s->tile_launch(t.nchunks, tt, is_spline);
+                    J.g = s->tile_launch(t.nchunks, tt, is_spline, s->target_fixed(t));
                 }
                 J.g.block = 1024;
                 J.g.chunks = t.chunk_order;
                 J.nchunks = t.nchunks;
                 J.flags = (s->arc ? kJobArc : 0) | (cplx ? kJobPhase : 0) | (lut ? kJobLut : 0) |
-                          (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0);
+                          (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0) |
+                          (J.g.fixed ? kJobFixed : 0);
                 const int tiles = (t.c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
                 J.blocks_per_slice = tiles * J.g.windows;
                 J.g.work = nullptr;
@@ -307,8 +308,8 @@ struct bcsim_batch {
                 groups.push_back(ProjGroup{vkey[j], (int)pitems.size(), 0, 0});
             ProjGroup& G = groups.back();
             const ProjJob& J = pjobs[j];
-            G.lds = std::max(G.lds, tile_lds_bytes(J.g.window_bins, (J.flags & kJobPhase) != 0, J.g.tile_lines,
-                                                   J.g.list_cap));
+            G.lds = std::max(G.lds, tile_lds_bytes(J.g.window_bins, tile_bin_bytes((J.flags & kJobPhase) != 0, J.g.fixed),
+                                                   J.g.tile_lines, J.g.list_cap));
             first_item[j] = pitems.size();
             const int n = J.blocks_per_slice * J.g.slices;
             for (int i = 0; i < n; i++) pitems.push_back(make_int2(j, i));

@@ -8,7 +8,8 @@
 //              bins spread over a window of 256 (about the radial spread of one
 //              chunk of 64 scatterers) whose start moves from step to step -- the
 //              pattern of the tile kernel's additions.  mode 1: 64 consecutive
-//              bins (one per bank), for comparison.
+//              bins (one per bank), for comparison.  mode 2: the same pattern with
+//              ds_add_u64 into 2 rows of 64-bit bins (the fixed-point bins).
 //   gather:    32-byte record gathers of a LUT of nr x 64 x 64 records (two
 //              16-byte loads each, as lut_weight reads a record), every wave's
 //              64 lanes in a span_r x span_l x span_e cell neighbourhood that
@@ -46,9 +47,12 @@ __device__ __forceinline__ unsigned hash32(unsigned x) {
     return x;
 }
 
+// T float (ds_add_f32) or unsigned long long (ds_add_u64); ROWS rows of kRowBins
+template <class T, int ROWS>
 __global__ __launch_bounds__(kThreads) void lds_add_kernel(int steps, int mode, float* sink) {
-    extern __shared__ float bins[];
-    for (int i = threadIdx.x; i < kRows * kRowBins; i += kThreads) bins[i] = 0.0f;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    T* bins = reinterpret_cast<T*>(lds_raw);
+    for (int i = threadIdx.x; i < ROWS * kRowBins; i += kThreads) bins[i] = T(0);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -56,14 +60,14 @@ __global__ __launch_bounds__(kThreads) void lds_add_kernel(int steps, int mode, 
     const unsigned lane_off = mode == 1 ? (unsigned)lane : hash32(lane * 0x85ebca6bu + blockIdx.x);
     for (int s = 0; s < steps; s++) {
         h = hash32(h + s);  // wave-uniform: row and window start of this step
-        const int row = h & (kRows - 1);
+        const int row = h & (ROWS - 1);
         const int start = (h >> 2) & 8191;
         const int bin = start + (int)((lane_off + (unsigned)s * 37u * (mode != 1)) & (kSpread - 1));
-        atomicAdd(&bins[row * kRowBins + bin], 1.0f);
+        atomicAdd(&bins[row * kRowBins + bin], T(1));
     }
     __syncthreads();
     float acc = 0.0f;
-    for (int i = threadIdx.x; i < kRows * kRowBins; i += kThreads) acc += bins[i];
+    for (int i = threadIdx.x; i < ROWS * kRowBins; i += kThreads) acc += (float)bins[i];
     sink[blockIdx.x * kThreads + threadIdx.x] = acc;
 }
 
@@ -143,12 +147,18 @@ float timed(Launch launch) {
 extern "C" {
 
 __attribute__((visibility("default"))) float ceil_lds_add(int workgroups, int steps, int mode, float* sink) {
-    const size_t lds = (size_t)kRows * kRowBins * sizeof(float);
-    if (hipFuncSetAttribute((const void*)lds_add_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-        hipSuccess)
-        return -1.0f;
+    if (mode < 0 || mode > 2) return -1.0f;
+    const bool wide = mode == 2;
+    const size_t lds = wide ? (size_t)2 * kRowBins * 8 : (size_t)kRows * kRowBins * sizeof(float);
+    const void* k = wide ? (const void*)lds_add_kernel<unsigned long long, 2> : (const void*)lds_add_kernel<float, kRows>;
+    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1.0f;
     return timed([&](hipStream_t s) {
-        hipLaunchKernelGGL(lds_add_kernel, dim3(workgroups), dim3(kThreads), lds, s, steps, mode, sink);
+        if (wide)
+            hipLaunchKernelGGL((lds_add_kernel<unsigned long long, 2>), dim3(workgroups), dim3(kThreads), lds, s, steps,
+                               0, sink);
+        else
+            hipLaunchKernelGGL((lds_add_kernel<float, kRows>), dim3(workgroups), dim3(kThreads), lds, s, steps, mode,
+                               sink);
     });
 }
 

@@ -21,18 +21,31 @@
 //       with a per-line hit mask to a list in LDS;
 //   (B) hands listed chunks to its waves, one chunk per wave and one scatterer
 //       per lane; each lane projects its scatterer onto every hit line and adds
-//       into the LDS bins with ds_add_f32.
+//       into the LDS bins.
 // The block leaves LDS once: plain stores when the workgroup saw every chunk,
 // otherwise atomic adds of the non-zero bins.
 //
+// LDS bins are float (ds_add_f32) or 64-bit fixed point (ds_add_u64, FIXED).
+// On gfx950 an LDS float atomic add retires about one lane per three cycles
+// while an integer one retires a wave's 64 lanes about ten times faster
+// (measured: ~2e11 against ~2e12 lane additions per second on the chip), so the
+// fixed-point bins take the additions off the kernel's critical unit at twice
+// the LDS per bin.  Each contribution is rounded once to the fixed-point grid
+// (ProjCommon::amp_scale; a quantum of about 2^-61 of the largest possible bin
+// sum) and the integer sums are exact, so a block's bins do not depend on the
+// order of its additions.
+//
 // Skipped pairs would have added w*a == 0 to a bin, so the sums equal those of
-// the direct evaluation up to float summation order.  When every line starts at
+// the direct evaluation up to float summation order (fixed point: up to the
+// rounding of each contribution to the grid).  When every line starts at
 // the same origin (sector and pyramidal scans) the arc-projected distance
 // |p - o|, its bin and the radial part of the LUT lookup are computed once per
 // scatterer instead of once per line.
 //
 // Compiled with -ffp-contract=off (see proj_math.h).
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "proj_math.h"
 #include "sim_device.h"
@@ -96,9 +109,30 @@ __device__ __forceinline__ bool chunk_hits_shared(const CullParams& cp, const Li
     return lateral_hits<LUT>(cp, L, B, dx, dy, dz, m);
 }
 
+// One LDS bin addition: float, or a contribution v already scaled by amp_scale
+// (|v| < 2^51 by the choice of amp_scale), rounded to the nearest integer and
+// added as a 64-bit two's-complement value.  The rounding and conversion are one
+// double addition: v + 1.5 * 2^52 lies in [2^52, 2^53), where doubles are the
+// integers, so the sum is 1.5 * 2^52 + rint(v) exactly, and its bit pattern minus
+// that of 1.5 * 2^52 is rint(v) (the low word of the constant is zero).
+__device__ __forceinline__ void lds_add(float* b, float v) { atomicAdd(b, v); }
+__device__ __forceinline__ void lds_add(long long* b, float v) {
+    const double t = (double)v + 6755399441055744.0;  // 1.5 * 2^52
+    const long long q = __double_as_longlong(t) - 0x4338000000000000LL;
+    atomicAdd(reinterpret_cast<unsigned long long*>(b), (unsigned long long)q);
+}
+
+// A bin sum as float.
+__device__ __forceinline__ float bin_value(float v, const ProjCommon&) { return v; }
+__device__ __forceinline__ float bin_value(long long v, const ProjCommon& c) {
+    return (float)((double)v * (double)c.inv_amp_scale);
+}
+
 // Weight times amplitude added at window-relative bin lb (CpuAlgorithm.cpp:77-95).
-template <bool PHASE>
-__device__ __forceinline__ void add_bin(const ProjCommon& c, float* lbins, int lb, int bin, double true_index,
+// With fixed-point bins the amplitude was scaled by amp_scale, a power of two, so
+// w is the float product w*a times amp_scale exactly.
+template <bool PHASE, class Acc>
+__device__ __forceinline__ void add_bin(const ProjCommon& c, Acc* lbins, int lb, int bin, double true_index,
                                         float w, Tally& t) {
     if (w == 0.0f) return;  // adding +-0 leaves the bin unchanged
     t.adds += PHASE ? 2 : 1;
@@ -106,24 +140,24 @@ __device__ __forceinline__ void add_bin(const ProjCommon& c, float* lbins, int l
         const float ph = (float)((double)bin - true_index) * c.phase_per_sample;
         float s, co;
         sincosf(ph, &s, &co);
-        atomicAdd(&lbins[2 * lb], w * co);
-        atomicAdd(&lbins[2 * lb + 1], w * s);
+        lds_add(&lbins[2 * lb], w * co);
+        lds_add(&lbins[2 * lb + 1], w * s);
     } else {
-        atomicAdd(&lbins[lb], w);
+        lds_add(&lbins[lb], w);
     }
 }
 
 // add_bin with the sub-sample rotation (cos, sin) of the phase already computed
 // (the same for every line when the lines share an origin).
-template <bool PHASE>
-__device__ __forceinline__ void add_bin_rot(float* lbins, int lb, float w, float co, float s, Tally& t) {
+template <bool PHASE, class Acc>
+__device__ __forceinline__ void add_bin_rot(Acc* lbins, int lb, float w, float co, float s, Tally& t) {
     if (w == 0.0f) return;
     t.adds += PHASE ? 2 : 1;
     if (PHASE) {
-        atomicAdd(&lbins[2 * lb], w * co);
-        atomicAdd(&lbins[2 * lb + 1], w * s);
+        lds_add(&lbins[2 * lb], w * co);
+        lds_add(&lbins[2 * lb + 1], w * s);
     } else {
-        atomicAdd(&lbins[lb], w);
+        lds_add(&lbins[lb], w);
     }
 }
 
@@ -136,8 +170,8 @@ __device__ __forceinline__ LineDirs line_dirs(const LineGeom& L) {
     return LineDirs{L.rx, L.ry, L.rz, L.lx, L.ly, L.lz, L.ex, L.ey, L.ez};
 }
 
-template <bool PHASE, bool LUT>
-__device__ __forceinline__ void emit(const ProjCommon& c, float* lbins, int b0, int nb, float r, float l, float e,
+template <bool PHASE, bool LUT, class Acc>
+__device__ __forceinline__ void emit(const ProjCommon& c, Acc* lbins, int b0, int nb, float r, float l, float e,
                                      int bin, double ti, float amp, Tally& t) {
     const int lb = bin - b0;
     if (bin < 0 || lb < 0 || lb >= nb) return;
@@ -166,27 +200,28 @@ __device__ __forceinline__ bool narrow_support(const ProfileParams& P, CullParam
 
 // One (tile, window, slice) block of work: everything a workgroup of the tile
 // kernel does.  `wg` = tile * windows + window; `lds` is the dynamic LDS and
-// `list_n` a workgroup-shared counter.  CP_AHEAD: a spline's active control
-// points are all loaded before the first is used (otherwise one control point
-// at a time).
-template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED, bool CP_AHEAD = true>
+// `list_n` a workgroup-shared counter.  FIXED: 64-bit fixed-point LDS bins.
+// CP_AHEAD: a spline's active control points are all loaded before the first is
+// used (otherwise one control point at a time).
+template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED, bool FIXED, bool CP_AHEAD = true>
 __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams& cp, const FixedSet& fs,
                                            const SplineSet& ss, const ChunkBox* __restrict__ boxes, int nchunks,
                                            float* __restrict__ out, const TileLaunch& g, int wg, int slice,
                                            float* lds, int& list_n) {
     constexpr int E = PHASE ? 2 : 1;
+    using Acc = typename std::conditional<FIXED, long long, float>::type;
     const int window = wg % g.windows;
     const int tile = wg / g.windows;
     const int line0 = tile * g.tile_lines;
     const int nl = min(g.tile_lines, c.num_lines - line0);
     const int b0 = window * g.window_bins;
     const int nb = min(g.window_bins, c.num_rf - b0);
-    const int stride = g.window_bins * E;  // LDS floats per line
+    const int stride = g.window_bins * E;  // LDS bins (Acc) per line
     // LDS: the tile's line geometry and spline basis (read for every hit; kept here
     // rather than re-read from global memory), then the bins, then the chunk list
     LineGeom* tl = reinterpret_cast<LineGeom*>(lds);
     SplineLineBasis* tb = reinterpret_cast<SplineLineBasis*>(tl + g.tile_lines);
-    float* bins = reinterpret_cast<float*>(tb + g.tile_lines);
+    Acc* bins = reinterpret_cast<Acc*>(tb + g.tile_lines);  // 32-byte aligned
     int* list_chunk = reinterpret_cast<int*>(bins + (size_t)g.tile_lines * stride);
     unsigned* list_mask = reinterpret_cast<unsigned*>(list_chunk + g.list_cap);
     {
@@ -199,7 +234,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
             for (int i = threadIdx.x; i < nl * 2; i += blockDim.x) bd[i] = bs[i];
         }
     }
-    zero_lds(bins, nl * stride);
+    for (int i = threadIdx.x; i < nl * stride; i += blockDim.x) bins[i] = Acc(0);
     if (threadIdx.x == 0) list_n = 0;
     __shared__ unsigned long long wg_stats[3];  // the block's launch statistics (TileLaunch::stats)
     if (g.stats && threadIdx.x < 3) wg_stats[threadIdx.x] = 0;
@@ -301,7 +336,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
             const int s = ci * kChunk + lane;
             if (s >= N) continue;
             if (SPLINE) {
-                const float amp = ss.amp[s];
+                const float amp = FIXED ? ss.amp[s] * c.amp_scale : ss.amp[s];
                 const size_t pitch = (size_t)N;
                 while (mask) {
                     const int k = __builtin_ctz(mask);
@@ -347,6 +382,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 }
             } else if (SHARED && ARC) {
                 const float4 p = fs.pts[s];
+                const float amp = FIXED ? p.w * c.amp_scale : p.w;
                 const float dx = p.x - org_x, dy = p.y - org_y, dz = p.z - org_z;
                 const float absd = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
                 double ti_pos;
@@ -366,12 +402,12 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                     if (!signbit(rp)) {
                         if (!in_pos) return;
                         const float w =
-                            (LUT ? lut_weight_le(c.prof, R, l, e, tally) : gauss_weight(c.prof, l, e, tally)) * p.w;
+                            (LUT ? lut_weight_le(c.prof, R, l, e, tally) : gauss_weight(c.prof, l, e, tally)) * amp;
                         add_bin_rot<PHASE>(bins + k * stride, lb_pos, w, co_pos, s_pos, tally);
                     } else {
                         double ti;
                         const int bin = radial_bin(c, -absd, ti);
-                        emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, p.w, tally);
+                        emit<PHASE, LUT>(c, bins + k * stride, b0, nb, -absd, l, e, bin, ti, amp, tally);
                     }
                 };
                 if (reg_lines) {
@@ -393,13 +429,14 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                 }
             } else {
                 const float4 p = fs.pts[s];
+                const float amp = FIXED ? p.w * c.amp_scale : p.w;
                 while (mask) {
                     const int k = __builtin_ctz(mask);
                     mask &= mask - 1;
                     const Local q = local_coords<ARC>(tl[k], p.x, p.y, p.z);
                     double ti;
                     const int bin = radial_bin(c, q.r, ti);
-                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, p.w, tally);
+                    emit<PHASE, LUT>(c, bins + k * stride, b0, nb, q.r, q.l, q.e, bin, ti, amp, tally);
                 }
             }
         }
@@ -431,7 +468,7 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     const int row = nb * E;
     for (int i = threadIdx.x; i < nl * row; i += blockDim.x) {
         const int k = i / row, o = i - k * row;
-        const float v = bins[k * stride + o];
+        const float v = bin_value(bins[k * stride + o], c);
         float* dst = out + ((size_t)(line0 + k) * c.num_rf + b0) * E + o;
         if (g.atomic) {
             if (v != 0.0f) unsafeAtomicAdd(dst, v);
@@ -441,14 +478,15 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     }
 }
 
-template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
+template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED, bool FIXED>
 __global__ __launch_bounds__(1024) void project_tile_kernel(ProjCommon c, CullParams cp, FixedSet fs, SplineSet ss,
                                                             const ChunkBox* __restrict__ boxes, int nchunks,
                                                             float* __restrict__ out, TileLaunch g) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ int list_n;
     const int wg = g.order ? g.order[blockIdx.x] : (int)blockIdx.x;
-    tile_block<ARC, PHASE, LUT, SPLINE, SHARED>(c, cp, fs, ss, boxes, nchunks, out, g, wg, blockIdx.y, lds, list_n);
+    tile_block<ARC, PHASE, LUT, SPLINE, SHARED, FIXED>(c, cp, fs, ss, boxes, nchunks, out, g, wg, blockIdx.y, lds,
+                                                       list_n);
 }
 
 // Order of the next tile launch's blocks: by the (chunk, line) pairs each listed
@@ -536,8 +574,8 @@ hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t s
 // Persistent projection over many jobs (simulators x datasets) of one kernel
 // variant: every workgroup takes (job, block) items from a global counter until
 // the list is exhausted, and runs the block.  KIND: 0 fixed points, 1 fixed
-// points with a shared origin (arc projection), 2 splines.
-template <bool ARC, bool PHASE, bool LUT, int KIND>
+// points with a shared origin (arc projection), 2 splines; FIXED: fixed-point bins.
+template <bool ARC, bool PHASE, bool LUT, int KIND, bool FIXED>
 __global__ __launch_bounds__(1024) void project_batch_kernel(const ProjJob* __restrict__ jobs,
                                                              const int2* __restrict__ items, int n_items,
                                                              int* __restrict__ counter) {
@@ -555,30 +593,32 @@ __global__ __launch_bounds__(1024) void project_batch_kernel(const ProjJob* __re
         const int blk = __builtin_amdgcn_readfirstlane(w.y);
         const int per_slice = J.blocks_per_slice;
         const int wg = blk % per_slice, slice = blk / per_slice;
-        tile_block<ARC, PHASE, LUT, KIND == 2, KIND == 1>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins, J.g, wg,
-                                                          slice, lds, list_n);
+        tile_block<ARC, PHASE, LUT, KIND == 2, KIND == 1, FIXED>(J.c, J.cp, J.fs, J.ss, J.boxes, J.nchunks, J.bins,
+                                                                 J.g, wg, slice, lds, list_n);
         __syncthreads();  // the next item reuses the LDS
     }
 }
 
-template <bool ARC, bool PHASE, bool LUT>
-static hipError_t launch_batch_kind(int kind, const ProjJob* jobs, const int2* items, int n_items, int* counter,
-                                    size_t lds, int workgroups, hipStream_t stream) {
-    const void* k = kind == 2   ? (const void*)project_batch_kernel<ARC, PHASE, LUT, 2>
-                    : kind == 1 ? (const void*)project_batch_kernel<ARC, PHASE, LUT, 1>
-                                : (const void*)project_batch_kernel<ARC, PHASE, LUT, 0>;
-    hipError_t e = allow_lds(k, lds);
+template <bool ARC, bool PHASE, bool LUT, int KIND, bool FIXED>
+static hipError_t launch_batch_t(const ProjJob* jobs, const int2* items, int n_items, int* counter, size_t lds,
+                                 int workgroups, hipStream_t stream) {
+    auto k = project_batch_kernel<ARC, PHASE, LUT, KIND, FIXED>;
+    hipError_t e = allow_lds((const void*)k, lds);
     if (e != hipSuccess) return e;
-    if (kind == 2)
-        hipLaunchKernelGGL((project_batch_kernel<ARC, PHASE, LUT, 2>), dim3(workgroups), dim3(1024), lds, stream, jobs,
-                           items, n_items, counter);
-    else if (kind == 1)
-        hipLaunchKernelGGL((project_batch_kernel<ARC, PHASE, LUT, 1>), dim3(workgroups), dim3(1024), lds, stream, jobs,
-                           items, n_items, counter);
-    else
-        hipLaunchKernelGGL((project_batch_kernel<ARC, PHASE, LUT, 0>), dim3(workgroups), dim3(1024), lds, stream, jobs,
-                           items, n_items, counter);
+    hipLaunchKernelGGL(k, dim3(workgroups), dim3(1024), lds, stream, jobs, items, n_items, counter);
     return hipGetLastError();
+}
+
+template <bool ARC, bool PHASE, bool LUT>
+static hipError_t launch_batch_kind(int kind, bool fixed, const ProjJob* jobs, const int2* items, int n_items,
+                                    int* counter, size_t lds, int workgroups, hipStream_t stream) {
+#define BCSIM_BATCH_K(K)                                                                                          \
+    return fixed ? launch_batch_t<ARC, PHASE, LUT, K, true>(jobs, items, n_items, counter, lds, workgroups, stream) \
+                 : launch_batch_t<ARC, PHASE, LUT, K, false>(jobs, items, n_items, counter, lds, workgroups, stream)
+    if (kind == 2) BCSIM_BATCH_K(2);
+    if (kind == 1) BCSIM_BATCH_K(1);
+    BCSIM_BATCH_K(0);
+#undef BCSIM_BATCH_K
 }
 
 hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* items, int n_items, int* counter,
@@ -586,7 +626,9 @@ hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* it
     if (n_items == 0) return hipSuccess;
     const bool arc = variant & kJobArc, phase = variant & kJobPhase, lut = variant & kJobLut;
     const int kind = (variant & kJobSpline) ? 2 : (arc && (variant & kJobShared)) ? 1 : 0;
-#define BCSIM_BATCH(A, P, U) return launch_batch_kind<A, P, U>(kind, jobs, items, n_items, counter, lds, workgroups, stream)
+    const bool fixed = variant & kJobFixed;
+#define BCSIM_BATCH(A, P, U) \
+    return launch_batch_kind<A, P, U>(kind, fixed, jobs, items, n_items, counter, lds, workgroups, stream)
     if (arc) {
         if (phase) {
             if (lut) BCSIM_BATCH(true, true, true);
@@ -604,22 +646,31 @@ hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* it
 #undef BCSIM_BATCH
 }
 
-size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap) {
-    return (size_t)tile_lines * ((size_t)window_bins * (complex_bins ? 8 : 4) + kTileLineLds) + (size_t)list_cap * 8;
+size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap) {
+    return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) + (size_t)list_cap * 8;
+}
+
+template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED, bool FIXED>
+static hipError_t launch_tile_f(const ProjCommon& c, const CullParams& cp, const FixedSet& fs, const SplineSet& ss,
+                                const ChunkBox* boxes, int nchunks, float* bins, const TileLaunch& cfg,
+                                hipStream_t stream) {
+    const int tiles = (c.num_lines + cfg.tile_lines - 1) / cfg.tile_lines;
+    const size_t lds = tile_lds_bytes(cfg.window_bins, tile_bin_bytes(PHASE, FIXED), cfg.tile_lines, cfg.list_cap);
+    auto k = project_tile_kernel<ARC, PHASE, LUT, SPLINE, SHARED, FIXED>;
+    hipError_t e = allow_lds((const void*)k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(tiles * cfg.windows, cfg.slices), dim3(cfg.block), lds, stream, c, cp, fs, ss, boxes,
+                       nchunks, bins, cfg);
+    return hipGetLastError();
 }
 
 template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED>
 static hipError_t launch_tile_t(const ProjCommon& c, const CullParams& cp, const FixedSet& fs, const SplineSet& ss,
                                 const ChunkBox* boxes, int nchunks, float* bins, const TileLaunch& cfg,
                                 hipStream_t stream) {
-    const int tiles = (c.num_lines + cfg.tile_lines - 1) / cfg.tile_lines;
-    const size_t lds = tile_lds_bytes(cfg.window_bins, PHASE, cfg.tile_lines, cfg.list_cap);
-    auto k = project_tile_kernel<ARC, PHASE, LUT, SPLINE, SHARED>;
-    hipError_t e = allow_lds((const void*)k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(tiles * cfg.windows, cfg.slices), dim3(cfg.block), lds, stream, c, cp, fs, ss, boxes,
-                       nchunks, bins, cfg);
-    return hipGetLastError();
+    if (cfg.fixed)
+        return launch_tile_f<ARC, PHASE, LUT, SPLINE, SHARED, true>(c, cp, fs, ss, boxes, nchunks, bins, cfg, stream);
+    return launch_tile_f<ARC, PHASE, LUT, SPLINE, SHARED, false>(c, cp, fs, ss, boxes, nchunks, bins, cfg, stream);
 }
 
 template <bool ARC, bool PHASE, bool LUT>

@@ -82,6 +82,13 @@ struct ProjCommon {
     // sub-sample phase: phase = (bin - true_index) * phase_per_sample, true_index = r*scale/denom
     float phase_per_sample;
     ProfileParams prof;
+    // Fixed-point bins (TileLaunch::fixed): a contribution w*a enters the LDS bins
+    // as the 64-bit integer rint(w * a * amp_scale), amp_scale = 2^k with every
+    // |w*a| * amp_scale below 2^51 and its sum over the launch's scatterers below
+    // 2^62, so no bin sum can overflow; a bin's sum is read back as
+    // float(sum * inv_amp_scale).
+    float amp_scale;
+    float inv_amp_scale;
 };
 
 struct FixedSet {
@@ -179,6 +186,7 @@ struct TileLaunch {
     int chunks_per_slice;
     int block;                // threads per workgroup (multiple of 64)
     int atomic;               // flush with global atomics
+    int fixed;                // LDS bins as 64-bit fixed point (ProjCommon::amp_scale), else float
     int list_cap;             // chunk list entries in LDS (multiple of 64)
     const int* order;         // workgroup w runs block order[w] (heaviest first), or null: block w
     int* work;                // per-block hit counts added up by the launch, or null
@@ -194,7 +202,7 @@ constexpr int kMaxOrderedTiles = 65536;
 
 // One projection job of a batch: a (simulator, dataset) pair with everything its
 // tile kernel would receive as arguments.
-enum JobFlags { kJobArc = 1, kJobPhase = 2, kJobLut = 4, kJobSpline = 8, kJobShared = 16 };
+enum JobFlags { kJobArc = 1, kJobPhase = 2, kJobLut = 4, kJobSpline = 8, kJobShared = 16, kJobFixed = 32 };
 
 struct ProjJob {
     ProjCommon c;
@@ -258,7 +266,9 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams& cp, const 
                                const SplineSet* spline, const ChunkBox* boxes, int nchunks, void* bins,
                                bool arc, bool phase, bool lut, bool shared_origin, const TileLaunch& cfg,
                                hipStream_t stream);
-size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap);
+// Bytes of one LDS bin of the tile kernel: float or 64-bit fixed point, real or complex.
+inline size_t tile_bin_bytes(bool complex_bins, bool fixed) { return (fixed ? 8 : 4) * (complex_bins ? 2 : 1); }
+size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap);
 // Orders blocks 0..n-1 (n <= kMaxOrderedTiles) by work[], heaviest first, into
 // order[] and clears work[] for the next tile launch to fill.
 hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream);
@@ -268,7 +278,7 @@ hipError_t launch_order_segments(const OrderSeg* segs, int n_segs, hipStream_t s
 // The JobFlags that select projection code: arc, phase, LUT, spline, and a
 // shared origin (which only matters for fixed points with arc projection).
 inline int batch_variant(int flags) {
-    int f = flags & (kJobArc | kJobPhase | kJobLut | kJobSpline | kJobShared);
+    int f = flags & (kJobArc | kJobPhase | kJobLut | kJobSpline | kJobShared | kJobFixed);
     if (!(f & kJobArc) || (f & kJobSpline)) f &= ~kJobShared;
     return f;
 }

@@ -124,6 +124,8 @@ struct FixedDataset {
     DevBuf boxes;  // ChunkBox[nchunks]
     size_t n = 0;
     int nchunks = 0;
+    double peak_amp = 0.0;  // largest |amplitude| (infinite if any is not finite)
+    double typ_amp = 0.0;   // median |amplitude| of the non-zero amplitudes
     mutable ChunkOrders visit;
 };
 
@@ -132,6 +134,8 @@ struct SplineDataset {
     int num_cp = 0;
     size_t n = 0;
     int nchunks = 0;
+    double peak_amp = 0.0;  // largest |amplitude| (infinite if any is not finite)
+    double typ_amp = 0.0;   // median |amplitude| of the non-zero amplitudes
     std::vector<float> knots;
     DevBuf cx, cy, cz, amp;  // [num_cp][n] x3, [n], Morton order of the control-point centroid
     DevBuf boxes;            // ChunkBox[nchunks] over all control points of the chunk
@@ -144,6 +148,55 @@ struct SplineScan {
     DevBuf rendered;  // float4[n], all-timestamps-equal path (culled with the dataset's boxes,
                       // since a rendered point lies in the box of its control points)
 };
+
+// Largest |a[i * stride]| over n values (infinite if one of them is not
+// finite) and the median |a| of the non-zero ones (a typical amplitude that a
+// few strong scatterers do not move).
+inline void amplitude_stats(const float* a, size_t n, size_t stride, double& peak, double& typical) {
+    std::vector<float> mag;
+    mag.reserve(n);
+    float m = 0.0f;
+    for (size_t i = 0; i < n; i++) {
+        const float v = std::fabs(a[i * stride]);
+        if (!(v <= 3.4e38f)) {
+            peak = typical = INFINITY;
+            return;
+        }
+        m = std::max(m, v);
+        if (v > 0.0f) mag.push_back(v);
+    }
+    peak = m;
+    typical = 0.0;
+    if (!mag.empty()) {
+        std::nth_element(mag.begin(), mag.begin() + mag.size() / 2, mag.end());
+        typical = mag[mag.size() / 2];
+    }
+}
+
+// Fixed-point scale of a projection launch (ProjCommon::amp_scale): the largest
+// power of two 2^k with peak * 2^k < 2^50 and peak * n * 2^k < 2^61, where peak
+// bounds every |w*a| of the launch (to float rounding) and n is its number of
+// scatterers: each scaled contribution is below 2^51 (the kernel's conversion
+// needs that) and no bin sum can overflow 64 bits.  The grid step 2^-k must also
+// stay within 2^-22 of a typical full-weight contribution (`typical`: the median
+// non-zero |a| times the profile's peak), else amplitudes span more than the
+// fixed point resolves (peak/typical beyond ~2^39/n) and the launch keeps float
+// bins: amp_scale 0, as for a peak that is not finite.
+inline void set_fixed_scale(ProjCommon& c, double peak, double typical, size_t n) {
+    c.amp_scale = 0.0f;
+    c.inv_amp_scale = 0.0f;
+    if (!std::isfinite(peak)) return;
+    int k = 0;
+    if (peak > 0.0) {
+        int e1 = 0, e2 = 0;
+        (void)std::frexp(peak, &e1);                                  // peak < 2^e1
+        (void)std::frexp(peak * (double)std::max<size_t>(n, 1), &e2);  // peak * n < 2^e2
+        k = std::max(-126, std::min({126, 50 - e1, 61 - e2}));
+        if (std::ldexp(1.0, -k) > std::ldexp(typical, -22)) return;
+    }
+    c.amp_scale = std::ldexp(1.0f, k);
+    c.inv_amp_scale = std::ldexp(1.0f, -k);
+}
 
 // 10 bits of each coordinate interleaved into a 30-bit z-curve key.
 inline uint32_t spread10(uint32_t x) {
@@ -318,6 +371,14 @@ struct bcsim_sim {
     // frame (counted by the launch itself); the order never changes the sums.
     bool tile_order = true;
     bool chunk_order = true;  // visit chunks near to far (ChunkOrders)
+    // bin_accumulation: the tile kernel's LDS bins as 64-bit fixed point ("fixed",
+    // integer LDS atomics; exact sums of contributions rounded to a grid of about
+    // 2^-61 of the launch's largest possible bin), float ("float", ds_add_f32), or
+    // "auto" (0): tuned where tuning is on, else fixed for the Gaussian profile and
+    // float for a LUT (target_fixed).  Datasets whose amplitudes the grid does not
+    // resolve (a non-finite amplitude, or a peak beyond ~2^39/n times the median;
+    // set_fixed_scale) use float bins.
+    int bin_mode = 0;  // 0 auto, 1 fixed, 2 float (TileTune::bins)
     std::vector<std::unique_ptr<DevBuf>> tile_order_bufs;  // per projection target: [work | order]
     int radix = 0;          // FFT pass radix set by the user (4 or 8); 0 = tuned or default
     std::string last_plan;  // launch shapes of the last frame, for get_parameter("launch_plan")
@@ -454,6 +515,15 @@ struct bcsim_sim {
             tile_order = parse_bool(value);
         } else if (key == "chunk_order") {
             chunk_order = parse_bool(value);
+        } else if (key == "bin_accumulation") {
+            if (value == "auto")
+                bin_mode = 0;
+            else if (value == "fixed")
+                bin_mode = 1;
+            else if (value == "float")
+                bin_mode = 2;
+            else
+                fail(BCSIM_ERR_INVALID_ARGUMENT, "bin_accumulation must be auto, fixed or float");
         } else if (key == "fft_radix") {
             const int v = parse_int(value);
             if (v != 0 && v != 4 && v != 8) fail(BCSIM_ERR_INVALID_ARGUMENT, "fft_radix must be 4, 8 or 0 (automatic)");
@@ -505,6 +575,7 @@ struct bcsim_sim {
         if (key == "autotune") return autotune ? "on" : "off";
         if (key == "tile_order") return tile_order ? "on" : "off";
         if (key == "chunk_order") return chunk_order ? "on" : "off";
+        if (key == "bin_accumulation") return bin_mode == 1 ? "fixed" : bin_mode == 2 ? "float" : "auto";
         if (key == "fft_radix") return std::to_string(radix);
         if (key == "launch_plan") return last_plan;
         if (key == "autotune_report") return TuneCache::get().report();
@@ -587,6 +658,7 @@ struct bcsim_sim {
         autotune = src.autotune;
         radix = src.radix;
         weight_cutoff = src.weight_cutoff;
+        bin_mode = src.bin_mode;
         have_exc = src.have_exc;
         exc = src.exc;
         center = src.center;
@@ -663,6 +735,7 @@ struct bcsim_sim {
         auto ds = std::make_shared<FixedDataset>();
         ds->n = n;
         ds->nchunks = (int)((n + kChunk - 1) / kChunk);
+        amplitude_stats(xyza + 3, n, 4, ds->peak_amp, ds->typ_amp);
         const std::vector<uint32_t> perm = morton_order(xyza, n, 4);
         std::vector<float> sorted(n * 4);
         for (size_t i = 0; i < n; i++) std::memcpy(&sorted[4 * i], xyza + 4 * (size_t)perm[i], 4 * sizeof(float));
@@ -709,6 +782,7 @@ struct bcsim_sim {
         ds->n = n;
         ds->knots.assign(knots, knots + n_knots);
         ds->nchunks = (int)((n + kChunk - 1) / kChunk);
+        amplitude_stats(amps, n, 1, ds->peak_amp, ds->typ_amp);
         // order by the centroid of each spline's control points
         std::vector<float> centroid(n * 3);
         for (size_t s = 0; s < n; s++) {
@@ -1200,10 +1274,10 @@ struct bcsim_sim {
     // slices for about 256 workgroups (1024 for splines with a time per line,
     // whose hits cost more: the tuner's choices on the 2D sector and the spline
     // M-mode, 6 % and 13 % faster than 512 workgroups).
-    TileLaunch tile_launch(int nchunks, const TileTune& tune, bool per_line_splines = false) const {
+    TileLaunch tile_launch(int nchunks, const TileTune& tune, bool per_line_splines, bool fixed_bins) const {
         TileLaunch cfg{};
         const int L = (int)lines.size();
-        const size_t bin_bytes = phase_delay ? 8 : 4;
+        const size_t bin_bytes = tile_bin_bytes(phase_delay, fixed_bins);
         const size_t avail = (size_t)kLdsBytes - 64 * 8 - 256;  // keep room for a 64-entry list
         // radial windows: one by default (measured fastest on the workloads of the tests
         // and bench); more if one line's bins (and its geometry, kTileLineLds) do not fit
@@ -1235,6 +1309,7 @@ struct bcsim_sim {
         cfg.slices = std::max(1, (nchunks + cfg.chunks_per_slice - 1) / cfg.chunks_per_slice);
         cfg.block = tune.block > 0 ? tune.block : 1024;
         cfg.atomic = cfg.slices > 1 ? 1 : 0;
+        cfg.fixed = fixed_bins ? 1 : 0;
         // the rest of the LDS holds the chunk list
         const size_t left = (size_t)kLdsBytes - 256 - (size_t)T * per_line;
         cfg.list_cap = (int)std::min<size_t>(kMaxListCap, (left / 8) / 64 * 64);
@@ -1255,6 +1330,21 @@ struct bcsim_sim {
         SplineScan* scan = nullptr;
         const int* chunk_order = nullptr;   // visiting order of the chunks (ChunkOrders), or null
     };
+
+    // Largest |weight| of the beam profile: 1 for the Gaussian, the largest |sample| of a LUT.
+    double profile_peak() const { return profile == kProfileLut ? (double)lut_peak : 1.0; }
+
+    // Does this target's launch use fixed-point bins?  Not where its amplitudes do
+    // not allow it (amp_scale 0); else the parameter if set, then `tuned` (a
+    // TileTune::bins), then the default: fixed for the Gaussian profile, whose LDS
+    // additions weigh on the kernel (fixed-point ones are ~10x cheaper), float for
+    // a LUT, whose table gathers bound it and where the wider bins only halve the
+    // lines a tile holds (measured on the 1M-point volume: 1.91 against 1.65 ms).
+    bool target_fixed(const ProjTarget& t, int tuned = 0) const {
+        if (!(t.c.amp_scale > 0.0f)) return false;
+        const int m = bin_mode ? bin_mode : tuned ? tuned : (profile == kProfileLut ? 2 : 1);
+        return m == 1;
+    }
 
     // Key of this scan's chunk visiting order (see ChunkOrders).
     ChunkOrders::Key chunk_order_key() const {
@@ -1296,6 +1386,7 @@ struct bcsim_sim {
         for (auto& f : fixed) {
             if (f->n == 0) continue;
             ProjTarget t{"fixed", proj_common(false)};
+            set_fixed_scale(t.c, f->peak_amp * profile_peak(), f->typ_amp * profile_peak(), f->n);
             t.fs = FixedSet{f->pts.as<float4>(), (int)f->n};
             t.boxes = f->boxes.as<ChunkBox>();
             t.nchunks = f->nchunks;
@@ -1308,6 +1399,7 @@ struct bcsim_sim {
             SplineScan* scan = spline_scan[q].get();
             if (sp->n == 0) continue;
             ProjTarget t{all_equal ? "rendered" : "spline", proj_common(true)};
+            set_fixed_scale(t.c, sp->peak_amp * profile_peak(), sp->typ_amp * profile_peak(), sp->n);
             t.ss = SplineSet{sp->cx.as<float>(), sp->cy.as<float>(), sp->cz.as<float>(), sp->amp.as<float>(),
                              scan->basis.as<SplineLineBasis>(), (int)sp->n, sp->num_cp};
             if (all_equal) {
@@ -1382,10 +1474,11 @@ struct bcsim_sim {
                   is_spline ? "spline projection kernel" : "fixed projection kernel");
     }
 
-    std::string shape_class(const char* kind, int nchunks) const {
+    std::string shape_class(const char* kind, int nchunks, bool fixed_ok) const {
         char buf[160];
-        std::snprintf(buf, sizeof buf, "tile/%s/%s/%s/%s/L2^%d/C2^%d/F%d", kind,
+        std::snprintf(buf, sizeof buf, "tile/%s/%s/%s/bins-%s/%s/L2^%d/C2^%d/F%d", kind,
                       profile == kProfileLut ? "lut" : "gauss", phase_delay ? "complex" : "real",
+                      !fixed_ok ? "float-only" : bin_mode == 1 ? "fixed" : bin_mode == 2 ? "float" : "auto",
                       arc ? "arc" : "noarc", ceil_log2((long long)lines.size()), ceil_log2(nchunks), fft_len);
         return buf;
     }
@@ -1395,7 +1488,8 @@ struct bcsim_sim {
     // workgroups, then block size), each candidate timed on this frame's data.
     // Writes the bins: call before they are cleared for the frame.
     TileTune tuned_tile(const ProjTarget& t) {
-        const std::string key = shape_class(t.kind, t.nchunks);
+        const bool fixed_ok = t.c.amp_scale > 0.0f;
+        const std::string key = shape_class(t.kind, t.nchunks, fixed_ok);
         TileTune best;
         if (TuneCache::get().find_tile(key, best)) return best;
         float* bins = d_bins.as<float>();
@@ -1406,7 +1500,7 @@ struct bcsim_sim {
         // blocks heaviest first by the counts of its own previous launch, so two warm
         // launches settle the order before three are timed
         auto cost = [&](const TileTune& tt) {
-            TileLaunch cfg = tile_launch(t.nchunks, tt, spl);
+            TileLaunch cfg = tile_launch(t.nchunks, tt, spl, target_fixed(t, tt.bins));
             const int blocks = tile_blocks(cfg);
             const bool ordered = tile_order && blocks >= kMinOrderedBlocks && blocks <= kMaxOrderedTiles;
             DevBuf order;
@@ -1425,20 +1519,32 @@ struct bcsim_sim {
                     launch_target(t, cfg, bins);
                 },
                 3, 2);
-            char b[48];
-            std::snprintf(b, sizeof b, " [%d,%d,%d,%d]=%.3f", cfg.tile_lines, cfg.block, cfg.slices, cfg.windows, ms);
+            char b[64];
+            std::snprintf(b, sizeof b, " [%d,%d,%d,%d,%s]=%.3f", cfg.tile_lines, cfg.block, cfg.slices, cfg.windows,
+                          cfg.fixed ? "fixed" : "float", ms);
             trials += b;
             return ms;
         };
+        // the bin accumulation first (unless set or not possible): each at its heuristic shape
+        best.bins = target_fixed(t) ? 1 : 2;
         double best_ms = cost(best);
-        const int tmax = tile_launch(t.nchunks, TileTune{0, 32, 0, 0}, spl).tile_lines;
+        if (fixed_ok && !bin_mode) {
+            TileTune c;
+            c.bins = best.bins == 1 ? 2 : 1;
+            const double ms = cost(c);
+            if (ms < best_ms) best = c, best_ms = ms;
+        }
+        const bool fx = target_fixed(t, best.bins);
+        TileTune wide;
+        wide.lines = 32;
+        const int tmax = tile_launch(t.nchunks, wide, spl, fx).tile_lines;
         for (int T = 1; T <= tmax; T *= 2) {
             TileTune c = best;
             c.lines = T;
             const double ms = cost(c);
             if (ms < best_ms) best = c, best_ms = ms;
         }
-        const TileLaunch cur = tile_launch(t.nchunks, best, spl);
+        const TileLaunch cur = tile_launch(t.nchunks, best, spl, fx);  // (best.bins stays as chosen)
         const int blocks = (((int)lines.size() + cur.tile_lines - 1) / cur.tile_lines) * cur.windows;
         for (int wgs : {256, 512, 1024, 2048, 4096}) {
             TileTune c = best;
@@ -1456,7 +1562,8 @@ struct bcsim_sim {
         }
         char head[64];
         std::snprintf(head, sizeof head, "%.3f ms with ", best_ms);
-        TuneCache::get().put_tile(key, best, head + describe(best) + "; trials [lines,block,slices,windows]:" + trials);
+        TuneCache::get().put_tile(key, best,
+                                  head + describe(best) + "; trials [lines,block,slices,windows,bins]:" + trials);
         return best;
     }
 
@@ -1470,8 +1577,9 @@ struct bcsim_sim {
             if (!eff.lines) eff.lines = tt.lines;
             if (!eff.block) eff.block = tt.block;
             if (!eff.slices) eff.slices = tt.slices;
+            eff.bins = tt.bins;
         }
-        return tile_launch(t.nchunks, eff, std::strcmp(t.kind, "spline") == 0);
+        return tile_launch(t.nchunks, eff, std::strcmp(t.kind, "spline") == 0, target_fixed(t, eff.bins));
     }
 
     // FFT pass radix of this frame's convolution: set by the user, tuned per
@@ -1543,7 +1651,8 @@ struct bcsim_sim {
                 const TileLaunch& g = plans.back();
                 last_plan += std::string(t.kind) + ": lines=" + std::to_string(g.tile_lines) +
                              " block=" + std::to_string(g.block) + " slices=" + std::to_string(g.slices) +
-                             " windows=" + std::to_string(g.windows) + "\n";
+                             " windows=" + std::to_string(g.windows) + " bins=" + (g.fixed ? "fixed" : "float") +
+                             "\n";
             }
         }
         const int frame_radix = plan_radix(autotune && !fm);

@@ -473,8 +473,14 @@ void check_tile(const ProjCommon& c, const FixedSet* fs, const SplineSet* ss, co
     if ((size_t)(g.slices - 1) * g.chunks_per_slice >= (size_t)nchunks) die(std::string(op) + ": empty slice");
     if (g.list_cap < 64 || g.list_cap > kMaxListCap || g.list_cap % 64) die(std::string(op) + ": list_cap");
     if (block < 64 || block > 1024 || block % 64) die(std::string(op) + ": block");
-    const size_t lds = tile_lds_bytes(g.window_bins, phase, g.tile_lines, g.list_cap);
+    const size_t lds = tile_lds_bytes(g.window_bins, tile_bin_bytes(phase, g.fixed), g.tile_lines, g.list_cap);
     if (lds + 8 > kLds) die(std::string(op) + ": LDS over 160 KiB");
+    if (g.fixed) {
+        // fixed-point bins: a power-of-two scale and its exact inverse
+        int e = 0;
+        if (!(c.amp_scale > 0.0f) || std::frexp(c.amp_scale, &e) != 0.5f || c.amp_scale * c.inv_amp_scale != 1.0f)
+            die(std::string(op) + ": fixed-point bins without a power-of-two amplitude scale");
+    }
     if (g.chunks) {
         check(g.chunks, (size_t)nchunks * 4, "chunk order", op);
         std::vector<char> seen(nchunks, 0);
@@ -519,8 +525,8 @@ void check_conv(const ConvParams& p, const char* op) {
 }
 }  // namespace
 
-size_t tile_lds_bytes(int window_bins, bool complex_bins, int tile_lines, int list_cap) {
-    return (size_t)tile_lines * ((size_t)window_bins * (complex_bins ? 8 : 4) + kTileLineLds) + (size_t)list_cap * 8;
+size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap) {
+    return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) + (size_t)list_cap * 8;
 }
 size_t convolve_batch_lds_bytes() { return ((size_t)kMaxFftLen + 2 * 128) * sizeof(float2); }
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
@@ -646,8 +652,11 @@ hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* it
                                      J.flags & kJobPhase, J.flags & kJobLut, J.g, 1024, op);
                           const int tiles = (J.c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
                           if (J.blocks_per_slice != tiles * J.g.windows) die("project_batch: blocks_per_slice");
-                          if (tile_lds_bytes(J.g.window_bins, J.flags & kJobPhase, J.g.tile_lines, J.g.list_cap) > lds)
+                          if (tile_lds_bytes(J.g.window_bins, tile_bin_bytes(J.flags & kJobPhase, J.g.fixed),
+                                             J.g.tile_lines, J.g.list_cap) > lds)
                               die("project_batch: job needs more LDS than launched");
+                          if (((J.flags & kJobFixed) != 0) != (J.g.fixed != 0))
+                              die("project_batch: job flags and launch disagree on fixed-point bins");
                       }
                       std::set<std::pair<int, int>> seen;
                       for (int i = 0; i < n_items; i++) {

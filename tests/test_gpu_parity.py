@@ -123,7 +123,10 @@ def test_iq_matches_reference_golden(name):
     bins = g.get_projection().ravel()
     ref = np.zeros_like(bins)
     ref[fx["bin_index"]] = fx["bin_value"]
-    strong = np.maximum(np.abs(bins), np.abs(ref)) > 1e-20
+    # the same bins are non-zero: every bin above 1e-9 of the largest (the
+    # fixed-point bins resolve contributions down to ~1e-15 of it; smaller sums
+    # are zero there while the reference's float tails reach 1e-40)
+    strong = np.maximum(np.abs(bins), np.abs(ref)) > 1e-9 * np.abs(ref).max()
     assert np.all(np.abs(ref[strong]) > 0) and np.all(np.abs(bins[strong]) > 0)
 
 
