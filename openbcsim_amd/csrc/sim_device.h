@@ -147,9 +147,16 @@ struct ConvParams {
 // LDS points (float2) of one line's convolution buffer; its twiddle table follows
 // it.  Real bins without noise, folded (fold_log2 >= 1): the N/2-point transform
 // of the packed samples and the folded inverse both fit N/2 points.
+// LDS slot of FFT point i: one float2 of padding after every 2^kConvPadShift points,
+// so that the early Stockham passes, whose lanes write 2^k points apart, spread
+// their writes over the banks.
+constexpr int kConvPadShift = 3;
+__host__ __device__ inline int conv_slot(int i) { return i + (i >> kConvPadShift); }
+
+// LDS float2 slots of the convolution's transform buffer (padded).
 __host__ __device__ inline int conv_buffer_len(const ConvParams& p) {
     const bool real_in = !p.complex_bins && !(p.noise_amplitude > 0.0f);
-    return (real_in && p.fold_log2 > 0) ? p.fft_len / 2 : p.fft_len;
+    return conv_slot((real_in && p.fold_log2 > 0) ? p.fft_len / 2 : p.fft_len);
 }
 
 // ---- culled tile projection (project_tile.hip) ----

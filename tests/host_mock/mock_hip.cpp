@@ -528,7 +528,7 @@ void check_conv(const ConvParams& p, const char* op) {
 size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap) {
     return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) + (size_t)list_cap * 8;
 }
-size_t convolve_batch_lds_bytes() { return ((size_t)kMaxFftLen + 2 * 128) * sizeof(float2); }
+size_t convolve_batch_lds_bytes() { return ((size_t)conv_slot(kMaxFftLen) + 2 * 128) * sizeof(float2); }
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
 
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s) {
