@@ -62,6 +62,12 @@ struct bcsim_batch {
         size_t lds;
     };
     std::vector<ProjGroup> groups;
+    // the convolution items of one kernel variant (conv_variant): a launch each
+    struct ConvGroup {
+        int variant, first, count;
+        size_t lds;  // the largest LDS any of its jobs needs
+    };
+    std::vector<ConvGroup> cgroups;
     struct Render {
         SplineSet set;
         SplineLineBasis basis;
@@ -315,13 +321,27 @@ struct bcsim_batch {
             for (int i = 0; i < n; i++) pitems.push_back(make_int2(j, i));
             G.count = (int)pitems.size() - G.first;
         }
+        // convolution items by kernel variant, longest transforms first within one; a
+        // launch per variant, its LDS that of its largest job
+        std::vector<int> cvar(cjobs.size());
+        for (size_t j = 0; j < cjobs.size(); j++)
+            cvar[j] = conv_variant(cjobs[j], conv_block_threads(cjobs[j].fft_len));
         std::vector<int> corder(cjobs.size());
         std::iota(corder.begin(), corder.end(), 0);
-        std::stable_sort(corder.begin(), corder.end(),
-                         [&](int a, int b) { return cjobs[a].fft_len > cjobs[b].fft_len; });
+        std::stable_sort(corder.begin(), corder.end(), [&](int a, int b) {
+            if (cvar[a] != cvar[b]) return cvar[a] > cvar[b];
+            return cjobs[a].fft_len > cjobs[b].fft_len;
+        });
         std::vector<int2> citems;
-        for (int j : corder)
+        cgroups.clear();
+        for (int j : corder) {
+            if (cgroups.empty() || cgroups.back().variant != cvar[j])
+                cgroups.push_back(ConvGroup{cvar[j], (int)citems.size(), 0, 0});
+            ConvGroup& C = cgroups.back();
+            C.lds = std::max(C.lds, conv_lds_bytes(cjobs[j]));
             for (int l = 0; l < cjobs[j].num_lines; l++) citems.push_back(make_int2(j, l));
+            C.count = (int)citems.size() - C.first;
+        }
 
         grow(d_proj_jobs, pjobs.size() * sizeof(ProjJob));
         grow(d_proj_items, pitems.size() * sizeof(int2));
@@ -329,7 +349,7 @@ struct bcsim_batch {
         grow(d_order_segs, ordered.size() * sizeof(OrderSeg));
         grow(d_conv_items, citems.size() * sizeof(int2));
         grow(d_conv_jobs, cjobs.size() * sizeof(ConvParams));
-        grow(d_counters, (groups.size() + 1) * sizeof(int));  // one per projection group, then the convolution's
+        grow(d_counters, std::max<size_t>(1, groups.size()) * sizeof(int));  // one per projection group
         grow(d_noise, cjobs.size() * sizeof(uint32_t));
         for (size_t k = 0; k < cjobs.size(); k++) cjobs[k].noise_counter_ptr = d_noise.as<uint32_t>() + k;
         std::vector<OrderSeg> order_segs;
@@ -380,7 +400,7 @@ struct bcsim_batch {
 
     void enqueue() {
         const int ng = (int)groups.size();
-        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), ng + 1,
+        hip_check(launch_fill_segments(d_fill.as<FillSeg>(), n_fill, d_counters.as<int>(), ng,
                                        d_noise.as<uint32_t>(), (int)plan.size(), stream),
                   "fill kernel");
         for (const Render& r : renders)
@@ -392,9 +412,12 @@ struct bcsim_batch {
                                            G.count, d_counters.as<int>() + g, G.lds, workgroups, stream),
                       "batched projection kernel");
         }
-        hip_check(launch_convolve_batch(d_conv_jobs.as<ConvParams>(), d_conv_items.as<int2>(), n_conv_items,
-                                        d_counters.as<int>() + ng, workgroups, stream),
-                  "batched convolution kernel");
+        for (size_t g = 0; g < cgroups.size(); g++) {
+            const ConvGroup& C = cgroups[g];
+            hip_check(launch_convolve_batch(C.variant, d_conv_jobs.as<ConvParams>(), d_conv_items.as<int2>() + C.first,
+                                            C.count, C.lds, stream),
+                      "batched convolution kernel");
+        }
     }
 
     void simulate_device(void* d_out, size_t capacity) {

@@ -290,16 +290,31 @@ inline int batch_variant(int flags) {
     return f;
 }
 
-// Batched frames: persistent workgroups take items from `counter` (reset by
+// LDS bytes of one line's convolution: the padded transform buffer and the
+// two-level twiddle table.
+__host__ __device__ inline size_t conv_lds_bytes(const ConvParams& p) {
+    return ((size_t)conv_buffer_len(p) + (size_t)(p.fft_len >> p.twiddle_bits) + ((size_t)1 << p.twiddle_bits)) *
+           sizeof(float2);
+}
+
+// Kernel variant of a batched convolution job: workgroup size class (1024 threads
+// for FFTs of 4096 points and more, else 256), radix and bin kind.
+enum ConvVariant { kConvComplex = 1, kConvRadix8 = 2, kConvWide = 4 };
+inline int conv_variant(const ConvParams& p, int block_threads) {
+    return (block_threads == 1024 ? kConvWide : 0) | (p.radix == 8 ? kConvRadix8 : 0) |
+           (p.complex_bins ? kConvComplex : 0);
+}
+
+// Batched frames.  launch_project_batch: persistent workgroups take the items of
+// one kernel variant (batch_variant of every item's job) from `counter` (reset by
 // launch_fill_segments, which also adds one to each of the n_noise per-job noise
-// frame counters).  launch_project_batch runs the items of one kernel variant
-// (batch_variant of every item's job).
+// frame counters).  launch_convolve_batch: one workgroup per item of one
+// convolution variant (conv_variant), `lds` bytes each.
 hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* items, int n_items, int* counter,
                                 size_t lds, int workgroups, hipStream_t stream);
-hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int n_items, int* counter,
-                                 int workgroups, hipStream_t stream);
+hipError_t launch_convolve_batch(int variant, const ConvParams* jobs, const int2* items, int n_items, size_t lds,
+                                 hipStream_t stream);
 hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, int n_counters,
                                 uint32_t* noise_counters, int n_noise, hipStream_t stream);
-size_t convolve_batch_lds_bytes();
 
 }  // namespace bcsim_hip

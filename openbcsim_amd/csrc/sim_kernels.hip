@@ -503,11 +503,13 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
         const float amp = p.noise_amplitude;
         const uint32_t seed = p.noise_seed;
         const uint32_t counter = p.noise_counter_ptr ? *p.noise_counter_ptr : p.noise_counter;
+        const float* bins_r = static_cast<const float*>(p.bins);
+        const float2* bins_c = static_cast<const float2*>(p.bins);
         const auto load = [=](int n) -> float2 {
             if (n >= nrf) return make_float2(0.f, 0.f);
             float2 v;
-            if (COMPLEX_BINS) v = static_cast<const float2*>(p.bins)[(size_t)line * nrf + n];
-            else v = make_float2(static_cast<const float*>(p.bins)[(size_t)line * nrf + n], 0.f);
+            if (COMPLEX_BINS) v = bins_c[(size_t)line * nrf + n];
+            else v = make_float2(bins_r[(size_t)line * nrf + n], 0.f);
             if (amp > 0.0f) {
                 const float2 z = normal_pair(seed, counter, (uint32_t)line, (uint32_t)n);
                 v = make_float2(v.x + amp * z.x, v.y + amp * z.y);
@@ -531,48 +533,42 @@ __global__ __launch_bounds__(NT) void convolve_demod_kernel(ConvParams p) {
     conv_line<NT, NMAX, COMPLEX_BINS, RMAX>(p, blockIdx.x, sbuf);
 }
 
-// Persistent convolution over the lines of many jobs: items are (job, line)
-// pairs taken from a global counter, longest transforms first.
-__global__ __launch_bounds__(1024) void convolve_batch_kernel(const ConvParams* __restrict__ jobs,
-                                                              const int2* __restrict__ items, int n_items,
-                                                              int* __restrict__ counter) {
+// Convolution of the lines of many jobs of one variant: workgroup b takes item
+// b, a (job, line) pair; items are ordered longest transforms first.  (A
+// persistent loop over the items was measured to keep the passes' per-thread
+// index arithmetic live across items: 128-160 VGPRs against 58-61 here.)
+template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX>
+__global__ __launch_bounds__(NT) void convolve_batch_kernel(const ConvParams* __restrict__ jobs,
+                                                            const int2* __restrict__ items) {
     extern __shared__ float2 sbuf[];
-    __shared__ int item_s;
-    for (;;) {
-        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
-        __syncthreads();
-        // item, job and line are wave-uniform: the job's parameters load into scalar registers
-        const int it = __builtin_amdgcn_readfirstlane(item_s);
-        __syncthreads();
-        if (it >= n_items) break;
-        const int2 w = items[it];
-        const ConvParams& p = jobs[__builtin_amdgcn_readfirstlane(w.x)];
-        const int line = __builtin_amdgcn_readfirstlane(w.y);
-        if (p.radix == 8) {
-            if (p.complex_bins) conv_line<1024, kMaxFftLen, true, 8>(p, line, sbuf);
-            else conv_line<1024, kMaxFftLen, false, 8>(p, line, sbuf);
-        } else {
-            if (p.complex_bins) conv_line<1024, kMaxFftLen, true, 4>(p, line, sbuf);
-            else conv_line<1024, kMaxFftLen, false, 4>(p, line, sbuf);
-        }
-        __syncthreads();
-    }
+    const int2 w = items[blockIdx.x];
+    conv_line<NT, NMAX, COMPLEX_BINS, RMAX>(jobs[w.x], w.y, sbuf);
 }
 
-size_t convolve_batch_lds_bytes() {
-    // the largest transform (padded) plus its two-level twiddle table (coarse and fine halves of log2 N)
-    return ((size_t)conv_slot(kMaxFftLen) + 2 * 128) * sizeof(float2);
-}
-
-hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int n_items, int* counter,
-                                 int workgroups, hipStream_t stream) {
-    if (n_items == 0) return hipSuccess;
-    const size_t lds = convolve_batch_lds_bytes();
-    hipError_t e = allow_lds((const void*)convolve_batch_kernel, lds);
+template <int NT, int NMAX, bool CB, int R>
+static hipError_t launch_conv_batch_t(const ConvParams* jobs, const int2* items, int n_items, size_t lds,
+                                      hipStream_t stream) {
+    auto k = convolve_batch_kernel<NT, NMAX, CB, R>;
+    hipError_t e = allow_lds((const void*)k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(convolve_batch_kernel, dim3(workgroups), dim3(1024), lds, stream, jobs, items, n_items,
-                       counter);
+    hipLaunchKernelGGL(k, dim3(n_items), dim3(NT), lds, stream, jobs, items);
     return hipGetLastError();
+}
+
+hipError_t launch_convolve_batch(int variant, const ConvParams* jobs, const int2* items, int n_items, size_t lds,
+                                 hipStream_t stream) {
+    if (n_items == 0) return hipSuccess;
+    const bool cb = variant & kConvComplex, r8 = variant & kConvRadix8;
+#define BCSIM_CONVB(NT, NMAX)                                                                           \
+    if (r8) return cb ? launch_conv_batch_t<NT, NMAX, true, 8>(jobs, items, n_items, lds, stream)       \
+                      : launch_conv_batch_t<NT, NMAX, false, 8>(jobs, items, n_items, lds, stream);     \
+    return cb ? launch_conv_batch_t<NT, NMAX, true, 4>(jobs, items, n_items, lds, stream)               \
+              : launch_conv_batch_t<NT, NMAX, false, 4>(jobs, items, n_items, lds, stream)
+    if (variant & kConvWide) {
+        BCSIM_CONVB(1024, kMaxFftLen);
+    }
+    BCSIM_CONVB(256, 4096);
+#undef BCSIM_CONVB
 }
 
 // Zero many regions, reset work counters and advance the jobs' noise frame
@@ -612,8 +608,7 @@ static hipError_t launch_conv(const ConvParams& p, size_t lds, hipStream_t strea
 hipError_t launch_convolve_demod(const ConvParams& p, hipStream_t stream) {
     if (p.num_lines == 0) return hipSuccess;
     const int nt = conv_block_threads(p.fft_len);
-    const size_t lds =
-        ((size_t)conv_buffer_len(p) + (p.fft_len >> p.twiddle_bits) + (1u << p.twiddle_bits)) * sizeof(float2);
+    const size_t lds = conv_lds_bytes(p);
     const bool cb = p.complex_bins != 0, r8 = p.radix == 8;
     if (nt == 1024) {
         if (r8) return cb ? launch_conv<1024, kMaxFftLen, true, 8>(p, lds, stream)

@@ -528,7 +528,6 @@ void check_conv(const ConvParams& p, const char* op) {
 size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap) {
     return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) + (size_t)list_cap * 8;
 }
-size_t convolve_batch_lds_bytes() { return ((size_t)conv_slot(kMaxFftLen) + 2 * 128) * sizeof(float2); }
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
 
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t s) {
@@ -673,17 +672,25 @@ hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* it
     return hipSuccess;
 }
 
-hipError_t launch_convolve_batch(const ConvParams* jobs, const int2* items, int n_items, int* counter, int,
+hipError_t launch_convolve_batch(int variant, const ConvParams* jobs, const int2* items, int n_items, size_t lds,
                                  hipStream_t s) {
     if (n_items == 0) return hipSuccess;
     enqueue(s, Op{[=] {
                       const char* op = "convolve_batch";
-                      check(counter, 4, "counter", op);
                       check(items, (size_t)n_items * sizeof(int2), "items", op);
+                      if (lds > kLds) die("convolve_batch: LDS over 160 KiB");
                       int nj = 0;
                       for (int i = 0; i < n_items; i++) nj = std::max(nj, items[i].x + 1);
                       check(jobs, (size_t)nj * sizeof(ConvParams), "jobs", op);
-                      for (int j = 0; j < nj; j++) check_conv(jobs[j], op);
+                      std::set<int> present;
+                      for (int i = 0; i < n_items; i++) present.insert(items[i].x);
+                      for (int j : present) {
+                          const ConvParams& p = jobs[j];
+                          check_conv(p, op);
+                          if (conv_variant(p, conv_block_threads(p.fft_len)) != variant)
+                              die("convolve_batch: item of another variant");
+                          if (conv_lds_bytes(p) > lds) die("convolve_batch: job needs more LDS than launched");
+                      }
                       for (int i = 0; i < n_items; i++)
                           if (items[i].y < 0 || items[i].y >= jobs[items[i].x].num_lines) die("convolve_batch: item");
                   },
