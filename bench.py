@@ -20,8 +20,9 @@ for a plain copy kernel of known size; achieved = traffic / duration against the
 8 TB/s HBM peak.  Beside it, the units that bind the kernel: its own counts of
 profile-weight evaluations (LUT record gathers) and LDS float additions per
 launch, as rates against ceilings measured on the device by kernels of the same
-access shape (openbcsim_amd/ceilings.py), and the per-line streaming model of the
-reference's kernel (16 B per scatterer evaluation) as a ratio to the peak.
+access shape (openbcsim_amd/ceilings.py), the busy fraction of the texture
+addresser and data units (a third counter run), and the per-line streaming model
+of the reference's kernel (16 B per scatterer evaluation) as a ratio to the peak.
 
 cpu_baseline: the reference's own CPU algorithm (CpuAlgorithm, compiled from the
 reference sources into oracle/_ref) on a bounded sample of the same workload --
@@ -631,7 +632,10 @@ def _pmc_values(out_dir: str):
 
 # counter passes of measure_traffic: FETCH_SIZE takes 3 TCC counters, WRITE_SIZE 2, so
 # they run apart; the SQ instruction counts ride along with the first
-PMC_PASSES = (("FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD"), ("WRITE_SIZE",))
+PMC_PASSES = (("FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD"), ("WRITE_SIZE",),
+              ("TD_TD_BUSY_sum", "TA_TA_BUSY_sum", "GRBM_GUI_ACTIVE"))
+# MI355X: GRBM_GUI_ACTIVE adds up the active cycles of its 8 accelerator complex dies
+XCDS = 8
 
 
 def measure_traffic(workload: str, keep_dir=None):
@@ -639,12 +643,15 @@ def measure_traffic(workload: str, keep_dir=None):
     rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE) on
     `workload`, each scaled by the counter's reading for copy_kernel moving
     COPY_CALIBRATION_BYTES; with the projection's VALU, LDS and vector-memory
-    instruction counts.  Returns (dict, None) or (None, reason)."""
+    instruction counts, and from a third run the busy cycles of the texture
+    addresser and data units beside the active cycles.  Returns (dict, None) or
+    (None, reason)."""
     exe = shutil.which("rocprofv3")
     if not exe:
         return None, "rocprofv3 not found"
     readings = {}
-    for counters in PMC_PASSES:
+    unit_err = None
+    for npass, counters in enumerate(PMC_PASSES):
         if keep_dir:
             os.makedirs(keep_dir, exist_ok=True)
         out_dir = os.path.abspath(tempfile.mkdtemp(prefix="pmc_", dir=keep_dir or tempfile.gettempdir()))
@@ -655,14 +662,21 @@ def measure_traffic(workload: str, keep_dir=None):
             r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240,
                                cwd=tempfile.gettempdir(), env=dict(os.environ, TMPDIR=tempfile.gettempdir()))
         except subprocess.TimeoutExpired:
+            if npass >= 2:  # the unit-busy pass is optional
+                unit_err = f"rocprofv3 --pmc {counters[0]} timed out"
+                continue
             return None, f"rocprofv3 --pmc {counters[0]} timed out"
         with open(os.path.join(out_dir, "run.log"), "w") as f:
             f.write(" ".join(cmd) + "\n" + r.stdout)
         vals = _pmc_values(out_dir)
         if counters[0] not in vals:
             files = sorted(os.path.relpath(p, out_dir) for p in glob.glob(os.path.join(out_dir, "**"), recursive=True))
-            return None, (f"rocprofv3 --pmc {counters[0]}: no counter values (rc {r.returncode}; files {files[:12]}; "
-                          f"output head {r.stdout[:600]!r} ... tail {r.stdout[-600:]!r})")
+            err = (f"rocprofv3 --pmc {counters[0]}: no counter values (rc {r.returncode}; files {files[:12]}; "
+                   f"output head {r.stdout[:600]!r} ... tail {r.stdout[-600:]!r})")
+            if npass >= 2:
+                unit_err = err
+                continue
+            return None, err
         readings.update(vals)
         if not keep_dir:
             shutil.rmtree(out_dir, ignore_errors=True)
@@ -684,20 +698,24 @@ def measure_traffic(workload: str, keep_dir=None):
             continue
         res[name] = {"fetch_kib": f, "write_kib": w,
                      "bytes": f * 1024.0 * scale["FETCH_SIZE"] + w * 1024.0 * scale["WRITE_SIZE"]}
-        for c in PMC_PASSES[0][1:]:
+        for c in PMC_PASSES[0][1:] + PMC_PASSES[2]:
             v = pick(c, substr)
             if v is not None:
                 res[name][c] = v
     res["scale"] = {"fetch": scale["FETCH_SIZE"], "write": scale["WRITE_SIZE"],
                     "calibration": f"copy_kernel moving {COPY_CALIBRATION_BYTES >> 20} MiB read + "
                                    f"{COPY_CALIBRATION_BYTES >> 20} MiB written"}
+    if unit_err:
+        res["unit_counters_error"] = unit_err
     return res, None
 
 
 def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings, fixed_bins=False):
     """The dominant kernel against HBM (measured bytes) and against the measured
-    ceilings of the units its counters show busy: LDS float atomics and LUT record
-    gathers (the launch's own counts) and VALU issue (SQ_INSTS_VALU)."""
+    ceilings of the units its counters show busy: LDS atomics and LUT record
+    gathers (the launch's own counts), VALU issue (SQ_INSTS_VALU), and the texture
+    addresser and data units (their busy cycles per CU over the kernel's active
+    cycles, from the counter run)."""
     N = wl.num_scatterers
     t = proj_ms * 1e-3
     rl = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": "project_tile_kernel",
@@ -736,6 +754,15 @@ def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings
             c = ceilings["copy_bytes_per_s"]
             units["hbm"] = {"per_launch": proj["bytes"], "achieved_per_s": proj["bytes"] / t, "ceiling_per_s": c,
                             "frac": round(proj["bytes"] / t / c, 4)}
+        if proj and proj.get("GRBM_GUI_ACTIVE") and ceilings.get("compute_units"):
+            # busy cycles of each CU's texture addresser / data unit against the kernel's
+            # active cycles per CU (GRBM_GUI_ACTIVE sums the XCDs' clocks)
+            active = proj["GRBM_GUI_ACTIVE"] / XCDS
+            for name, key in (("texture_addresser", "TA_TA_BUSY_sum"), ("texture_data", "TD_TD_BUSY_sum")):
+                if proj.get(key):
+                    units[name] = {"busy_cycles": proj[key], "active_cycles_per_cu": active,
+                                   "compute_units": ceilings["compute_units"],
+                                   "frac": round(proj[key] / (ceilings["compute_units"] * active), 4)}
         rl["units"] = units
         rl["ceilings"] = {k: round(v, 1) for k, v in ceilings.items()}
         if units:
@@ -792,6 +819,7 @@ def main():
     proj_ms = kern.get(projection_key(wl))
     from openbcsim_amd import ceilings as CE
     ceil = CE.measure()
+    ceil["compute_units"] = CE.cu_count()
     log(f"ceilings: {ceil}")
     lds_bins = counts.pop("bins", None)
     roofline = (roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceil,

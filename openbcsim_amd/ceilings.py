@@ -38,6 +38,8 @@ def load():
         lib.ceil_valu.argtypes = [C.c_int, C.c_int, C.c_void_p]
         lib.ceil_copy.restype = C.c_float
         lib.ceil_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong]
+        lib.ceil_cu_count.restype = C.c_int
+        lib.ceil_cu_count.argtypes = []
         _lib = lib
     return _lib
 
@@ -85,6 +87,13 @@ def copy_rate(nbytes: int = 1 << 30) -> float:
     src.close()
     dst.close()
     return 2.0 * nbytes / (ms * 1e-3)
+
+
+def cu_count() -> int:
+    n = load().ceil_cu_count()
+    if n < 1:
+        raise RuntimeError("compute unit count query failed")
+    return int(n)
 
 
 def measure() -> dict:

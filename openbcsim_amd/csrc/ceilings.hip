@@ -29,6 +29,7 @@
 //                     int workgroups, int steps, float* d_sink)
 //   float ceil_valu(int workgroups, int iters, float* d_sink)
 //   float ceil_copy(float4* d_dst, const float4* d_src, long long n4)
+//   int   ceil_cu_count(void)   (compute units of the current device, or -1)
 #include <hip/hip_runtime.h>
 
 namespace {
@@ -180,6 +181,13 @@ __attribute__((visibility("default"))) float ceil_valu(int workgroups, int iters
 
 __attribute__((visibility("default"))) float ceil_copy(float4* dst, const float4* src, long long n4) {
     return timed([&](hipStream_t s) { hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, dst, src, n4); });
+}
+
+__attribute__((visibility("default"))) int ceil_cu_count() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    return cus;
 }
 
 }  // extern "C"
