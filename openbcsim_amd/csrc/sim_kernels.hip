@@ -287,10 +287,15 @@ __device__ __forceinline__ void dft8(const float2* x, float2* y) {
 // i goes to dst(i, value).  Ls is the length of the sub-transforms already
 // formed.  With SYNC every thread reads all its inputs before the workgroup
 // synchronises and any thread writes (a pass in place over one LDS buffer).
-template <int R, int NT, int NMAX, bool INV, bool SYNC, class Src, class Dst>
-__device__ __forceinline__ void stockham_io(int N, int Ls, const Twiddle& tw, const Src& src, const Dst& dst) {
+// CN, CLS: N and Ls as compile-time constants (0: the run-time arguments), which
+// turns the index arithmetic of the pass into constants and LDS offset fields.
+template <int R, int NT, int NMAX, bool INV, bool SYNC, int CN = 0, int CLS = 0, class Src, class Dst>
+__device__ __forceinline__ void stockham_io(int N_rt, int Ls_rt, const Twiddle& tw, const Src& src,
+                                            const Dst& dst) {
 #pragma clang fp contract(fast)
-    constexpr int IT = (NMAX / R + NT - 1) / NT;
+    const int N = CN ? CN : N_rt;
+    const int Ls = CLS ? CLS : Ls_rt;
+    constexpr int IT = ((CN ? CN : NMAX) / R + NT - 1) / NT;
     const int m = N / R;
     float2 x[IT][R];
 #pragma unroll
@@ -397,6 +402,57 @@ __device__ __forceinline__ void fft_fused(float2* buf, int n, int log2n, const T
     if (SYNCL) __syncthreads();
 }
 
+// fft_fused for a transform length 2^LOG2N fixed at compile time: the same
+// passes, each with its N and Ls constants.
+template <int RMAX>
+constexpr int first_radix_c(int log2n) {
+    return RMAX == 8 ? (log2n % 3 == 1 ? 2 : log2n % 3 == 2 ? 4 : 8) : ((log2n & 1) ? 2 : 4);
+}
+
+template <int NT, int NMAX, bool INV, int RMAX, bool SYNCL, int N, int LS, class Dst>
+__device__ __forceinline__ void fft_passes_c(float2* buf, const Twiddle& tw, const Dst& dstL) {
+    const auto lds_src = [buf](int i) { return buf[conv_slot(i)]; };
+    if constexpr (LS * RMAX < N) {
+        const auto lds_dst = [buf](int i, float2 v) { buf[conv_slot(i)] = v; };
+        stockham_io<RMAX, NT, NMAX, INV, true, N, LS>(N, LS, tw, lds_src, lds_dst);
+        __syncthreads();
+        fft_passes_c<NT, NMAX, INV, RMAX, SYNCL, N, LS * RMAX>(buf, tw, dstL);
+    } else {
+        stockham_io<RMAX, NT, NMAX, INV, SYNCL, N, LS>(N, LS, tw, lds_src, dstL);
+        if (SYNCL) __syncthreads();
+    }
+}
+
+template <int NT, int NMAX, bool INV, int RMAX, bool SYNC0, bool SYNCL, int LOG2N, class Src, class Dst>
+__device__ __forceinline__ void fft_fused_c(float2* buf, const Twiddle& tw, const Src& src0, const Dst& dstL) {
+    constexpr int n = 1 << LOG2N;
+    constexpr int r0 = first_radix_c<RMAX>(LOG2N);
+    if constexpr (n == r0) {  // a single pass
+        stockham_io<r0, NT, NMAX, INV, SYNC0 || SYNCL, n, 1>(n, 1, tw, src0, dstL);
+        if (SYNCL) __syncthreads();
+    } else {
+        const auto lds_dst = [buf](int i, float2 v) { buf[conv_slot(i)] = v; };
+        stockham_io<r0, NT, NMAX, INV, SYNC0, n, 1>(n, 1, tw, src0, lds_dst);
+        __syncthreads();
+        fft_passes_c<NT, NMAX, INV, RMAX, SYNCL, n, r0>(buf, tw, dstL);
+    }
+}
+
+// The transform of n = 2^log2n points: the lengths of the volume, sector and
+// spline workloads (2^11 .. 2^13) with constant passes, others through fft_fused.
+template <int NT, int NMAX, bool INV, int RMAX, bool SYNC0, bool SYNCL, class Src, class Dst>
+__device__ __forceinline__ void fft_any(float2* buf, int n, int log2n, const Twiddle& tw, const Src& src0,
+                                        const Dst& dstL) {
+    if (NMAX >= 8192 && log2n == 13)
+        fft_fused_c<NT, NMAX, INV, RMAX, SYNC0, SYNCL, 13>(buf, tw, src0, dstL);
+    else if (NMAX >= 4096 && log2n == 12)
+        fft_fused_c<NT, NMAX, INV, RMAX, SYNC0, SYNCL, 12>(buf, tw, src0, dstL);
+    else if (log2n == 11)
+        fft_fused_c<NT, NMAX, INV, RMAX, SYNC0, SYNCL, 11>(buf, tw, src0, dstL);
+    else
+        fft_fused<NT, NMAX, INV, RMAX, SYNC0, SYNCL>(buf, n, log2n, tw, src0, dstL);
+}
+
 // Counter-based normal variates for the noise path: a 32-bit integer hash
 // (Wang's mix, then a multiply-xorshift round) feeds a Box-Muller transform.
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -478,7 +534,7 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
             const int n = 2 * m;
             return make_float2(n < nrf ? src[n] : 0.f, n + 1 < nrf ? src[n + 1] : 0.f);
         };
-        fft_fused<NT, NMAX, false, RMAX, false, true>(buf, half, p.log2_len - 1,
+        fft_any<NT, NMAX, false, RMAX, false, true>(buf, half, p.log2_len - 1,
                                                      Twiddle{twc, twf, p.twiddle_bits, n_fine - 1, 1}, load,
                                                      [buf](int i, float2 v) { buf[conv_slot(i)] = v; });
         // X[k] H[k] from Z[k] and Z[N/2 - k]
@@ -495,8 +551,8 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
             return acc;
         };
         const auto one_sided = [=](int k) -> float2 { return k <= half ? spec(k) : make_float2(0.f, 0.f); };
-        if (p.fold_log2 == 0) fft_fused<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, one_sided, store);
-        else fft_fused<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, fold, store);
+        if (p.fold_log2 == 0) fft_any<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, one_sided, store);
+        else fft_any<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, fold, store);
     } else {
         // complex bins (phase delay) or noise: the full N-point transform
         const int nrf = p.num_rf;
@@ -516,14 +572,14 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
             }
             return v;
         };
-        fft_fused<NT, NMAX, false, RMAX, false, true>(buf, N, p.log2_len, twN, load,
+        fft_any<NT, NMAX, false, RMAX, false, true>(buf, N, p.log2_len, twN, load,
                                                      [buf](int i, float2 v) { buf[conv_slot(i)] = v; });
         const auto fold = [=](int j) -> float2 {
             float2 acc = make_float2(0.f, 0.f);
             for (int k = j; k <= half; k += M) acc = cadd(acc, cmul(buf[conv_slot(k)], H[k]));
             return acc;
         };
-        fft_fused<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, fold, store);
+        fft_any<NT, NMAX, true, RMAX, true, false>(buf, M, log2m, twM, fold, store);
     }
 }
 
