@@ -439,11 +439,18 @@ __device__ __forceinline__ void fft_fused_c(float2* buf, const Twiddle& tw, cons
 }
 
 // The transform of n = 2^log2n points: the lengths of the volume, sector and
-// spline workloads (2^11 .. 2^13) with constant passes, others through fft_fused.
-template <int NT, int NMAX, bool INV, int RMAX, bool SYNC0, bool SYNCL, class Src, class Dst>
+// spline workloads (2^11 .. 2^13, and with W14 2^14) with constant passes, others
+// through fft_fused.  W14 only in a kernel for complex transforms of 2^14 points,
+// which runs one workgroup per CU anyway (its buffer takes most of the LDS): the
+// 2^14-point passes hold more points per thread, and elsewhere their registers
+// would cost the second workgroup per CU (measured: the volume's real-bin
+// convolution 0.165 -> 0.222 ms with them in its kernel).
+template <int NT, int NMAX, bool INV, int RMAX, bool SYNC0, bool SYNCL, bool W14 = false, class Src, class Dst>
 __device__ __forceinline__ void fft_any(float2* buf, int n, int log2n, const Twiddle& tw, const Src& src0,
                                         const Dst& dstL) {
-    if (NMAX >= 8192 && log2n == 13)
+    if (W14 && NMAX >= 16384 && log2n == 14)
+        fft_fused_c<NT, NMAX, INV, RMAX, SYNC0, SYNCL, 14>(buf, tw, src0, dstL);
+    else if (NMAX >= 8192 && log2n == 13)
         fft_fused_c<NT, NMAX, INV, RMAX, SYNC0, SYNCL, 13>(buf, tw, src0, dstL);
     else if (NMAX >= 4096 && log2n == 12)
         fft_fused_c<NT, NMAX, INV, RMAX, SYNC0, SYNCL, 12>(buf, tw, src0, dstL);
@@ -498,7 +505,7 @@ __device__ __forceinline__ float2 normal_pair(uint32_t seed, uint32_t counter, u
 // workgroup barrier and writes after it, so it runs in place.  The buffer is
 // conv_buffer_len(p) points (N/2 for real bins folded by F >= 2), the twiddle
 // table follows it.
-template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX>
+template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX, bool W14 = false>
 __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2* sbuf) {
 #pragma clang fp contract(fast)
     const int N = p.fft_len;
@@ -572,7 +579,7 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
             }
             return v;
         };
-        fft_any<NT, NMAX, false, RMAX, false, true>(buf, N, p.log2_len, twN, load,
+        fft_any<NT, NMAX, false, RMAX, false, true, W14>(buf, N, p.log2_len, twN, load,
                                                      [buf](int i, float2 v) { buf[conv_slot(i)] = v; });
         const auto fold = [=](int j) -> float2 {
             float2 acc = make_float2(0.f, 0.f);
@@ -583,10 +590,10 @@ __device__ __forceinline__ void conv_line(const ConvParams& p, int line, float2*
     }
 }
 
-template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX>
+template <int NT, int NMAX, bool COMPLEX_BINS, int RMAX, bool W14 = false>
 __global__ __launch_bounds__(NT) void convolve_demod_kernel(ConvParams p) {
     extern __shared__ float2 sbuf[];
-    conv_line<NT, NMAX, COMPLEX_BINS, RMAX>(p, blockIdx.x, sbuf);
+    conv_line<NT, NMAX, COMPLEX_BINS, RMAX, W14>(p, blockIdx.x, sbuf);
 }
 
 // Convolution of the lines of many jobs of one variant: workgroup b takes item
@@ -652,12 +659,12 @@ hipError_t launch_fill_segments(const FillSeg* segs, int n_segs, int* counters, 
 
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
 
-template <int NT, int NMAX, bool CB, int R>
+template <int NT, int NMAX, bool CB, int R, bool W14 = false>
 static hipError_t launch_conv(const ConvParams& p, size_t lds, hipStream_t stream) {
-    const void* k = (const void*)convolve_demod_kernel<NT, NMAX, CB, R>;
+    const void* k = (const void*)convolve_demod_kernel<NT, NMAX, CB, R, W14>;
     hipError_t e = allow_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((convolve_demod_kernel<NT, NMAX, CB, R>), dim3(p.num_lines), dim3(NT), lds, stream, p);
+    hipLaunchKernelGGL((convolve_demod_kernel<NT, NMAX, CB, R, W14>), dim3(p.num_lines), dim3(NT), lds, stream, p);
     return hipGetLastError();
 }
 
@@ -667,6 +674,9 @@ hipError_t launch_convolve_demod(const ConvParams& p, hipStream_t stream) {
     const size_t lds = conv_lds_bytes(p);
     const bool cb = p.complex_bins != 0, r8 = p.radix == 8;
     if (nt == 1024) {
+        if (cb && p.fft_len == 16384)  // the complex 2^14-point transform's own kernel (fft_any)
+            return r8 ? launch_conv<1024, kMaxFftLen, true, 8, true>(p, lds, stream)
+                      : launch_conv<1024, kMaxFftLen, true, 4, true>(p, lds, stream);
         if (r8) return cb ? launch_conv<1024, kMaxFftLen, true, 8>(p, lds, stream)
                           : launch_conv<1024, kMaxFftLen, false, 8>(p, lds, stream);
         return cb ? launch_conv<1024, kMaxFftLen, true, 4>(p, lds, stream)
