@@ -309,6 +309,11 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
                             if (chunk_hits_shared<LUT>(cpc, tl[k], B, dx, dy, dz, m, pos_ok, neg_ok)) mask |= 1u << k;
                     }
                 }
+            } else if (SPLINE && ss.iboxes) {
+                // each line against the box of the chunk's control points on its knot interval
+                const ChunkBox* ib = ss.iboxes + (size_t)ci * ss.n_iv;
+                for (int k = 0; k < nl; k++)
+                    if (chunk_hits<ARC, LUT>(cp, r_lo, r_hi, tl[k], ib[tb[k].lo])) mask |= 1u << k;
             } else {
                 for (int k = 0; k < nl; k++)
                     if (chunk_hits<ARC, LUT>(cp, r_lo, r_hi, tl[k], B)) mask |= 1u << k;
@@ -678,7 +683,7 @@ static hipError_t launch_tile_kind(const ProjCommon& c, const CullParams& cp, co
                                    const SplineSet* spline, const ChunkBox* boxes, int nchunks, float* bins,
                                    bool shared, const TileLaunch& cfg, hipStream_t stream) {
     const FixedSet fs = fixed ? *fixed : FixedSet{nullptr, 0};
-    const SplineSet ss = spline ? *spline : SplineSet{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+    const SplineSet ss = spline ? *spline : SplineSet{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, 0};
     if (spline) return launch_tile_t<ARC, PHASE, LUT, true, false>(c, cp, fs, ss, boxes, nchunks, bins, cfg, stream);
     if (ARC && shared)
         return launch_tile_t<ARC, PHASE, LUT, false, true>(c, cp, fs, ss, boxes, nchunks, bins, cfg, stream);

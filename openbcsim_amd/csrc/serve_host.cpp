@@ -230,7 +230,8 @@ struct bcsim_server {
         if (s.lut) count(s.lut.get(), s.lut->bytes);
         for (auto& f : s.fixed) count(f.get(), f->pts.bytes + f->boxes.bytes);
         for (auto& sp : s.splines)
-            count(sp.get(), sp->cx.bytes + sp->cy.bytes + sp->cz.bytes + sp->amp.bytes + sp->boxes.bytes);
+            count(sp.get(), sp->cx.bytes + sp->cy.bytes + sp->cz.bytes + sp->amp.bytes + sp->boxes.bytes +
+                                sp->iboxes.bytes);
         if (used + add > budget) fail(BCSIM_ERR_OUT_OF_MEMORY, "phantom datasets exceed the server's memory budget");
         resident_sets.insert(added.begin(), added.end());
         resident += add;

@@ -96,6 +96,8 @@ struct FixedSet {
     int n;
 };
 
+struct ChunkBox;  // (below)
+
 struct SplineSet {
     const float* cx;
     const float* cy;
@@ -104,6 +106,12 @@ struct SplineSet {
     const SplineLineBasis* basis;   // [num_lines]
     int n;
     int num_cp;
+    // or null: per chunk and knot interval, the box of the chunk's active control
+    // points on that interval ([nchunks][n_iv], n_iv = num_cp - degree); a spline
+    // point on a line lies in the box of its line's interval (the basis values are
+    // non-negative and sum to one), which is tighter than the box of all its points
+    const ChunkBox* iboxes;
+    int n_iv;
 };
 
 struct ProjLaunch {

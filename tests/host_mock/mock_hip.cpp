@@ -442,6 +442,10 @@ void check_common(const ProjCommon& c, bool lut, const char* op) {
 
 void check_spline(const SplineSet& s, int num_lines, bool basis, const char* op) {
     const size_t cp = (size_t)s.num_cp * s.n * 4;
+    if (s.iboxes) {
+        if (s.n_iv < 1 || s.n_iv > s.num_cp) die(std::string(op) + ": knot interval count");
+        check(s.iboxes, (size_t)((s.n + kChunk - 1) / kChunk) * s.n_iv * sizeof(ChunkBox), "interval boxes", op);
+    }
     check(s.cx, cp, "cx", op);
     check(s.cy, cp, "cy", op);
     check(s.cz, cp, "cz", op);
@@ -452,6 +456,7 @@ void check_spline(const SplineSet& s, int num_lines, bool basis, const char* op)
         memcpy(b.data(), s.basis, b.size() * sizeof(SplineLineBasis));
         for (auto& x : b)
             if (x.lo < 0 || x.count < 1 || x.lo + x.count > s.num_cp) die(std::string(op) + ": basis outside control points");
+            else if (s.iboxes && x.lo >= s.n_iv) die(std::string(op) + ": basis interval outside the interval boxes");
     }
 }
 
