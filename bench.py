@@ -135,7 +135,7 @@ def run_gpu(wl, steps: int, warmup: int, tile=None, bins=None):
 
 def run_configs(steps: int = 5):
     """Every single-simulator config with phase delay off and on (device-resident IQ,
-    frame time over `steps` frames after one warm-up): the 2D sector (100k points,
+    frame time over `steps` frames after three warm-up frames): the 2D sector (100k points,
     128 lines), the spline M-mode (200k cubic splines, 256 lines, one time per
     line) and the 1M-point volume, with the library's choice of LDS bins; each
     again with fixed-point and with float LDS bins forced, and with weight_cutoff 0
@@ -158,7 +158,8 @@ def run_configs(steps: int = 5):
                                     ("cutoff_off", {"bin_accumulation": "auto", "weight_cutoff": "0"})):
                 for key, val in params.items():
                     sim.set_parameter(key, val)
-                sim.simulate_lines_device(buf.address, L * S)
+                for _ in range(3):  # the block order of a launch shape settles after its first frame
+                    sim.simulate_lines_device(buf.address, L * S)
                 sim.synchronize()
                 t0 = time.perf_counter()
                 for _ in range(steps):
