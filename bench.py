@@ -135,7 +135,7 @@ def run_gpu(wl, steps: int, warmup: int, tile=None, bins=None):
 
 def run_configs(steps: int = 5):
     """Every single-simulator config with phase delay off and on (device-resident IQ,
-    frame time over `steps` frames after three warm-up frames): the 2D sector (100k points,
+    frame time over `steps` frames after at least 0.1 s of warm-up frames): the 2D sector (100k points,
     128 lines), the spline M-mode (200k cubic splines, 256 lines, one time per
     line) and the 1M-point volume, with the library's choice of LDS bins; each
     again with fixed-point and with float LDS bins forced, and with weight_cutoff 0
@@ -158,8 +158,15 @@ def run_configs(steps: int = 5):
                                     ("cutoff_off", {"bin_accumulation": "auto", "weight_cutoff": "0"})):
                 for key, val in params.items():
                     sim.set_parameter(key, val)
-                for _ in range(3):  # the block order of a launch shape settles after its first frame
+                # warm-up: at least 3 frames (the block order of a launch shape settles after
+                # its first) and 0.1 s of frames (after the idle host-side set-up of a new
+                # simulator the first ~10 frames run ~2 % slower)
+                n_warm, t_warm = 0, time.perf_counter() + 0.1
+                while n_warm < 3 or time.perf_counter() < t_warm:
                     sim.simulate_lines_device(buf.address, L * S)
+                    n_warm += 1
+                    if n_warm % 4 == 0:
+                        sim.synchronize()
                 sim.synchronize()
                 t0 = time.perf_counter()
                 for _ in range(steps):
