@@ -720,8 +720,8 @@ def measure_traffic(workload: str, keep_dir=None):
 
 def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings, fixed_bins=False):
     """The dominant kernel against HBM (measured bytes) and against the measured
-    ceilings of the units its counters show busy: LDS atomics and LUT record
-    gathers (the launch's own counts), VALU issue (SQ_INSTS_VALU), and the texture
+    ceilings of the units its counters show busy: LDS atomics, LUT record
+    gathers and spline control-point loads (the launch's own counts), VALU issue (SQ_INSTS_VALU), and the texture
     addresser and data units (their busy cycles per CU over the kernel's active
     cycles, from the counter run)."""
     N = wl.num_scatterers
@@ -755,6 +755,12 @@ def roofline_object(wl, kind, L, proj_ms, counts, traffic, traffic_err, ceilings
             c = ceilings["lut_gathers_per_s"]
             units["lut_record_gathers"] = {"per_launch": evals, "achieved_per_s": evals / t, "ceiling_per_s": c,
                                            "frac": round(evals / t / c, 4)}
+        pairs = (counts or {}).get("projection_chunk_line_pairs")
+        if pairs and kind == "spline" and ceilings.get("spline_wave_hits_per_s"):
+            # each listed (chunk, line) pair is one wave hit: its splines' active control points
+            c = ceilings["spline_wave_hits_per_s"]
+            units["spline_control_point_loads"] = {"per_launch": pairs, "achieved_per_s": pairs / t,
+                                                   "ceiling_per_s": c, "frac": round(pairs / t / c, 4)}
         if proj and proj.get("SQ_INSTS_VALU"):
             v, c = proj["SQ_INSTS_VALU"], ceilings["valu_wave_instructions_per_s"]
             units["valu"] = {"per_launch": v, "achieved_per_s": v / t, "ceiling_per_s": c, "frac": round(v / t / c, 4)}

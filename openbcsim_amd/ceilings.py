@@ -9,6 +9,9 @@ libbcsim_ceilings.so:
 * ``gather_rate``: 32-byte LUT record gathers per second, each wave's lanes in
   a chunk-sized neighbourhood of an L2-resident table (the projection's LUT
   lookups with nothing else in the way);
+* ``spline_rate``: spline hits of one wave per second from the control-point
+  loads alone (4 active control points of 64 consecutive splines of a
+  200k x 10 dataset, [cp][n] float arrays: 12 coalesced loads per lane);
 * ``valu_rate``: VALU wave instructions per second (independent FMA chains);
 * ``copy_rate``: bytes moved per second by a plain float4 copy (read + write).
 
@@ -34,6 +37,9 @@ def load():
         lib.ceil_lds_add.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p]
         lib.ceil_gather.restype = C.c_float
         lib.ceil_gather.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        lib.ceil_spline.restype = C.c_float
+        lib.ceil_spline.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_void_p]
         lib.ceil_valu.restype = C.c_float
         lib.ceil_valu.argtypes = [C.c_int, C.c_int, C.c_void_p]
         lib.ceil_copy.restype = C.c_float
@@ -71,6 +77,20 @@ def gather_rate(slices: int = 32, span=(5, 10, 10), workgroups: int = 1024, step
     return workgroups * 1024.0 * steps / (ms * 1e-3)
 
 
+def spline_rate(n: int = 200_000, m: int = 10, workgroups: int = 1024, steps: int = 512) -> float:
+    """Wave spline hits per second: per wave and step, the 4 active control points
+    of 64 consecutive splines (random chunk and knot interval) of an n x m dataset."""
+    from .hip_rt import DeviceBuffer
+    arrays = [DeviceBuffer(n * m * 4) for _ in range(3)]
+    sink = DeviceBuffer(workgroups * 1024 * 4)
+    ms = _ms(load().ceil_spline(*(C.c_void_p(a.address) for a in arrays), n, m, workgroups, steps,
+                                C.c_void_p(sink.address)), "spline")
+    for a in arrays:
+        a.close()
+    sink.close()
+    return workgroups * 16.0 * steps / (ms * 1e-3)
+
+
 def valu_rate(workgroups: int = 1024, iters: int = 4096) -> float:
     """VALU wave instructions per second: 8 independent FMA chains per thread."""
     from .hip_rt import DeviceBuffer
@@ -103,5 +123,6 @@ def measure() -> dict:
             "lds_int64_atomics_per_s": lds_add_rate(mode=2),
             "lut_gathers_per_s": gather_rate(),
             "lut_gathers_per_s_one_line_per_lane": gather_rate(span=(32, 64, 64)),
+            "spline_wave_hits_per_s": spline_rate(),
             "valu_wave_instructions_per_s": valu_rate(),
             "copy_bytes_per_s": copy_rate()}

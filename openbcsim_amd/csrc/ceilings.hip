@@ -17,6 +17,13 @@
 //              and a 5 x 10 x 10 span (64 scatterers of a chunk, ~3 mm apart, on
 //              the volume workload's table), the rate of the projection's LUT
 //              gathers without anything else in the way.
+//   spline:    the control-point loads of a spline hit: every step, each wave
+//              reads the 4 active control points (cubic) of 64 consecutive
+//              splines at a random chunk and knot interval from the [cp][n] float
+//              arrays x, y, z of an n x m dataset (12 coalesced 4-byte loads per
+//              lane) and evaluates the position: spline hits of one wave per
+//              second, the per-line spline projection's loads with nothing else
+//              in the way.
 //   valu:      independent FMA chains: VALU wave instructions per second.
 //   copy:      a plain float4 copy (the calibration kernel of the HBM-traffic
 //              counters: its bytes are known).
@@ -26,6 +33,8 @@
 // negative value on a HIP error:
 //   float ceil_lds_add(int workgroups, int steps, int mode, float* d_sink)
 //   float ceil_gather(const float4* d_table, int nr, int span_r, int span_l, int span_e,
+//                     int workgroups, int steps, float* d_sink)
+//   float ceil_spline(const float* d_cx, const float* d_cy, const float* d_cz, int n, int m,
 //                     int workgroups, int steps, float* d_sink)
 //   float ceil_valu(int workgroups, int iters, float* d_sink)
 //   float ceil_copy(float4* d_dst, const float4* d_src, long long n4)
@@ -100,6 +109,35 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(const float4* __restri
     sink[blockIdx.x * kThreads + threadIdx.x] = acc;
 }
 
+// A spline hit's control-point loads: a random chunk of 64 splines (n is a
+// multiple of 64) and a random interval of 4 active control points per wave and
+// step, position summed with fixed basis values.
+__global__ __launch_bounds__(kThreads) void spline_kernel(const float* __restrict__ cx, const float* __restrict__ cy,
+                                                          const float* __restrict__ cz, int n, int m, int steps,
+                                                          float* sink) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    unsigned h = hash32((blockIdx.x * 16u + wave) * 0x9e3779b9u);
+    const unsigned nchunks = (unsigned)(n / 64), intervals = (unsigned)(m - 3);
+    float acc = 0.0f;
+    for (int s = 0; s < steps; s++) {
+        h = hash32(h + s);  // wave-uniform: chunk and interval of this step
+        const size_t base = (size_t)((h >> 20) % intervals) * n + (size_t)(h % nchunks) * 64 + lane;
+        float x[4], y[4], z[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            x[i] = cx[base + (size_t)i * n];
+            y[i] = cy[base + (size_t)i * n];
+            z[i] = cz[base + (size_t)i * n];
+        }
+        const float px = x[0] * 0.1f + x[1] * 0.4f + x[2] * 0.4f + x[3] * 0.1f;
+        const float py = y[0] * 0.1f + y[1] * 0.4f + y[2] * 0.4f + y[3] * 0.1f;
+        const float pz = z[0] * 0.1f + z[1] * 0.4f + z[2] * 0.4f + z[3] * 0.1f;
+        acc += px * py + pz;
+    }
+    sink[blockIdx.x * kThreads + threadIdx.x] = acc;
+}
+
 // Independent FMA chains: the VALU's wave-instruction issue rate.
 __global__ __launch_bounds__(kThreads) void valu_kernel(int iters, float* sink) {
     float a[8];
@@ -170,6 +208,14 @@ __attribute__((visibility("default"))) float ceil_gather(const float4* table, in
     return timed([&](hipStream_t s) {
         hipLaunchKernelGGL(gather_kernel, dim3(workgroups), dim3(kThreads), 0, s, table, nr, span_r, span_l, span_e,
                            steps, sink);
+    });
+}
+
+__attribute__((visibility("default"))) float ceil_spline(const float* cx, const float* cy, const float* cz, int n,
+                                                         int m, int workgroups, int steps, float* sink) {
+    if (n < 64 || n % 64 != 0 || m < 4) return -1.0f;
+    return timed([&](hipStream_t s) {
+        hipLaunchKernelGGL(spline_kernel, dim3(workgroups), dim3(kThreads), 0, s, cx, cy, cz, n, m, steps, sink);
     });
 }
 

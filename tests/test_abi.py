@@ -84,3 +84,14 @@ def test_c_header_compiles_as_c99_and_links():
                                   INCLUDE, "-"], input='#include "bcsim_c.h"\nint main(void) { return 0; }\n',
                                  capture_output=True, text=True)
             assert res.returncode == 0, (std, res.stderr)
+
+
+def test_ceiling_probes_library_binds():
+    """The bench's ceiling probes (libbcsim_ceilings.so) load and export every
+    entry point the binding declares; nothing is launched without a device."""
+    from openbcsim_amd import ceilings
+    if not os.path.exists(ceilings.LIB_PATH):
+        pytest.skip("libbcsim_ceilings.so is not built")
+    lib = ceilings.load()
+    for name in ("ceil_lds_add", "ceil_gather", "ceil_spline", "ceil_valu", "ceil_copy", "ceil_cu_count"):
+        assert getattr(lib, name).restype is not None
