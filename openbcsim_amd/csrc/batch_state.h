@@ -315,7 +315,7 @@ struct bcsim_batch {
             ProjGroup& G = groups.back();
             const ProjJob& J = pjobs[j];
             G.lds = std::max(G.lds, tile_lds_bytes(J.g.window_bins, tile_bin_bytes((J.flags & kJobPhase) != 0, J.g.fixed),
-                                                   J.g.tile_lines, J.g.list_cap));
+                                                   J.g.tile_lines, J.g.list_cap, J.g.list_packed));
             first_item[j] = pitems.size();
             const int n = J.blocks_per_slice * J.g.slices;
             for (int i = 0; i < n; i++) pitems.push_back(make_int2(j, i));

@@ -223,7 +223,8 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     SplineLineBasis* tb = reinterpret_cast<SplineLineBasis*>(tl + g.tile_lines);
     Acc* bins = reinterpret_cast<Acc*>(tb + g.tile_lines);  // 32-byte aligned
     int* list_chunk = reinterpret_cast<int*>(bins + (size_t)g.tile_lines * stride);
-    unsigned* list_mask = reinterpret_cast<unsigned*>(list_chunk + g.list_cap);
+    unsigned* list_mask = reinterpret_cast<unsigned*>(list_chunk + g.list_cap);  // two-word entries
+    unsigned* list_word = reinterpret_cast<unsigned*>(list_chunk);              // packed entries
     {
         const float4* src = reinterpret_cast<const float4*>(c.lines + line0);
         float4* dst = reinterpret_cast<float4*>(tl);
@@ -321,8 +322,12 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
             {
                 if (mask) {
                     const int slot = atomicAdd(&list_n, 1);
-                    list_chunk[slot] = ci;
-                    list_mask[slot] = mask;
+                    if (g.list_packed) {
+                        list_word[slot] = ((unsigned)ci << 4) | mask;
+                    } else {
+                        list_chunk[slot] = ci;
+                        list_mask[slot] = mask;
+                    }
                     hits += __popc(mask);
                 }
             }
@@ -336,8 +341,16 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
         for (int j = wave; j < n; j += nwaves) {
             // one chunk per wave: its index and hit mask are wave-uniform, so the
             // loop over hit lines is scalar and the line data is a broadcast LDS read
-            const int ci = __builtin_amdgcn_readfirstlane(list_chunk[j]);
-            unsigned mask = __builtin_amdgcn_readfirstlane(list_mask[j]);
+            int ci;
+            unsigned mask;
+            if (g.list_packed) {
+                const unsigned w = __builtin_amdgcn_readfirstlane(list_word[j]);
+                ci = (int)(w >> 4);
+                mask = w & 15u;
+            } else {
+                ci = __builtin_amdgcn_readfirstlane(list_chunk[j]);
+                mask = __builtin_amdgcn_readfirstlane(list_mask[j]);
+            }
             const int s = ci * kChunk + lane;
             if (s >= N) continue;
             if (SPLINE) {
@@ -651,8 +664,9 @@ hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* it
 #undef BCSIM_BATCH
 }
 
-size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap) {
-    return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) + (size_t)list_cap * 8;
+size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap, int list_packed) {
+    return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) +
+           (size_t)list_cap * (list_packed ? 4 : 8);
 }
 
 template <bool ARC, bool PHASE, bool LUT, bool SPLINE, bool SHARED, bool FIXED>
@@ -660,7 +674,8 @@ static hipError_t launch_tile_f(const ProjCommon& c, const CullParams& cp, const
                                 const ChunkBox* boxes, int nchunks, float* bins, const TileLaunch& cfg,
                                 hipStream_t stream) {
     const int tiles = (c.num_lines + cfg.tile_lines - 1) / cfg.tile_lines;
-    const size_t lds = tile_lds_bytes(cfg.window_bins, tile_bin_bytes(PHASE, FIXED), cfg.tile_lines, cfg.list_cap);
+    const size_t lds =
+        tile_lds_bytes(cfg.window_bins, tile_bin_bytes(PHASE, FIXED), cfg.tile_lines, cfg.list_cap, cfg.list_packed);
     auto k = project_tile_kernel<ARC, PHASE, LUT, SPLINE, SHARED, FIXED>;
     hipError_t e = allow_lds((const void*)k, lds);
     if (e != hipSuccess) return e;

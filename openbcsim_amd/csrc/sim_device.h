@@ -203,6 +203,8 @@ struct TileLaunch {
     int atomic;               // flush with global atomics
     int fixed;                // LDS bins as 64-bit fixed point (ProjCommon::amp_scale), else float
     int list_cap;             // chunk list entries in LDS (multiple of 64)
+    int list_packed;          // 1: an entry is one 32-bit word, chunk << 4 | line mask (tiles of at most
+                              // 4 lines, fewer than 2^28 chunks); 0: two words, chunk and mask
     const int* order;         // workgroup w runs block order[w] (heaviest first), or null: block w
     int* work;                // per-block hit counts added up by the launch, or null
     const int* chunks;        // chunk visiting order (a permutation of [0, nchunks)), or null: index order
@@ -283,7 +285,7 @@ hipError_t launch_project_tile(const ProjCommon& c, const CullParams& cp, const 
                                hipStream_t stream);
 // Bytes of one LDS bin of the tile kernel: float or 64-bit fixed point, real or complex.
 inline size_t tile_bin_bytes(bool complex_bins, bool fixed) { return (fixed ? 8 : 4) * (complex_bins ? 2 : 1); }
-size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap);
+size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap, int list_packed);
 // Orders blocks 0..n-1 (n <= kMaxOrderedTiles) by work[], heaviest first, into
 // order[] and clears work[] for the next tile launch to fill.
 hipError_t launch_order_tiles(int* work, int* order, int n, hipStream_t stream);

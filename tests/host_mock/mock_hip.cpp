@@ -478,7 +478,9 @@ void check_tile(const ProjCommon& c, const FixedSet* fs, const SplineSet* ss, co
     if ((size_t)(g.slices - 1) * g.chunks_per_slice >= (size_t)nchunks) die(std::string(op) + ": empty slice");
     if (g.list_cap < 64 || g.list_cap > kMaxListCap || g.list_cap % 64) die(std::string(op) + ": list_cap");
     if (block < 64 || block > 1024 || block % 64) die(std::string(op) + ": block");
-    const size_t lds = tile_lds_bytes(g.window_bins, tile_bin_bytes(phase, g.fixed), g.tile_lines, g.list_cap);
+    if (g.list_packed && (g.tile_lines > 4 || nchunks >= (1 << 28))) die(std::string(op) + ": list_packed");
+    const size_t lds =
+        tile_lds_bytes(g.window_bins, tile_bin_bytes(phase, g.fixed), g.tile_lines, g.list_cap, g.list_packed);
     if (lds + 8 > kLds) die(std::string(op) + ": LDS over 160 KiB");
     if (g.fixed) {
         // fixed-point bins: a power-of-two scale and its exact inverse
@@ -530,8 +532,9 @@ void check_conv(const ConvParams& p, const char* op) {
 }
 }  // namespace
 
-size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap) {
-    return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) + (size_t)list_cap * 8;
+size_t tile_lds_bytes(int window_bins, size_t bin_bytes, int tile_lines, int list_cap, int list_packed) {
+    return (size_t)tile_lines * ((size_t)window_bins * bin_bytes + kTileLineLds) +
+           (size_t)list_cap * (list_packed ? 4 : 8);
 }
 int conv_block_threads(int fft_len) { return fft_len >= 4096 ? 1024 : 256; }
 
@@ -657,7 +660,7 @@ hipError_t launch_project_batch(int variant, const ProjJob* jobs, const int2* it
                           const int tiles = (J.c.num_lines + J.g.tile_lines - 1) / J.g.tile_lines;
                           if (J.blocks_per_slice != tiles * J.g.windows) die("project_batch: blocks_per_slice");
                           if (tile_lds_bytes(J.g.window_bins, tile_bin_bytes(J.flags & kJobPhase, J.g.fixed),
-                                             J.g.tile_lines, J.g.list_cap) > lds)
+                                             J.g.tile_lines, J.g.list_cap, J.g.list_packed) > lds)
                               die("project_batch: job needs more LDS than launched");
                           if (((J.flags & kJobFixed) != 0) != (J.g.fixed != 0))
                               die("project_batch: job flags and launch disagree on fixed-point bins");
