@@ -237,6 +237,16 @@ int main(int argc, char** argv) {
         CHK(bcsim_set_parameter(v.sim, "store_kernel_details", "on"));
         for (int i = 0; i < 3; i++) CHK(bcsim_simulate_lines_device(v.sim, d_out, (size_t)L * S, &L, &S));
         CHK(bcsim_set_parameter(v.sim, "store_kernel_details", "off"));
+        {
+            // 4-line tiles of float bins leave the LDS room for 896 two-word list entries:
+            // the plan holds the list as one-word entries instead
+            char plan[512];
+            CHK(bcsim_get_parameter(v.sim, "launch_plan", plan, sizeof plan));
+            if (!std::strstr(plan, "lines=4") || !std::strstr(plan, "list=1792 packed")) {
+                fprintf(stderr, "FAIL volume launch plan: %s\n", plan);
+                exit(1);
+            }
+        }
         std::vector<float> host((size_t)L * S * 2);
         for (int i = 0; i < 3; i++) CHK(bcsim_simulate_lines(v.sim, host.data(), (size_t)L * S, &L, &S));
         CHK(bcsim_destroy(v.sim));

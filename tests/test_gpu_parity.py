@@ -271,17 +271,25 @@ def test_tiled_matches_direct_projection(name):
     assert rel_l2(out["tiled"][0], out["direct"][0]) < 1e-6
 
 
-def test_tiled_matches_direct_full_volume():
-    wl = W.volume_lut()
-    res = {}
+@pytest.mark.parametrize("phase", [False, True])
+def test_tiled_matches_direct_full_volume(phase):
+    """The tiled kernel against the direct one (every pair, no culling, no chunk
+    list) on the full volume, whose tiles (4 lines of real bins, 2 of complex
+    ones) hold the chunk list as one-word entries (chunk << 4 | line mask)."""
+    wl = W.volume_lut(phase_delay=phase)
+    res, plan = {}, ""
     for mode in ("direct", "tiled"):
         g = gpu_sim()
         g.set_parameter("projection_mode", mode)
         wl.apply(g)
         res[mode] = g.simulate_frame()
+        if mode == "tiled":
+            plan = g.get_parameter("launch_plan")
         g.close()
     err = rel_l2(res["tiled"], res["direct"])
-    print(f"[result] full volume tiled vs direct rel_l2={err:.3e}", flush=True)
+    print(f"[result] full volume (phase delay {phase}) tiled vs direct rel_l2={err:.3e}; plan {plan.strip()!r}",
+          flush=True)
+    assert ("lines=2" if phase else "lines=4") in plan and "packed" in plan
     assert err < 1e-6
 
 
