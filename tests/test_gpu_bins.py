@@ -184,3 +184,18 @@ def test_automatic_choice():
     print(f"[result] tuner: {line}", flush=True)
     assert line and ",fixed]=" in line[0] and ",float]=" in line[0]
     assert rel_l2(iq, oracle_frame(wl)) <= 1e-5
+
+
+def test_full_phase_volume_tuned_shape_matches_default():
+    """The shape the tuner picks for the full 1M-point volume with phase delay
+    (fixed-point complex bins, 4 radial windows of 4 lines, one-word chunk-list
+    entries) against the default shape (float bins, whole 2-line tiles): equal
+    up to float summation order."""
+    wl = W.volume_lut(phase_delay=True)
+    ref, plan_ref = gpu_frame(wl, "auto")
+    iq, plan = gpu_frame(wl, "fixed", tile_windows=4, tile_lines=4)
+    e = rel_l2(iq, ref)
+    print(f"[result] tuned shape {plan.strip()!r} vs default {plan_ref.strip()!r}: rel_l2 {e:.2e}", flush=True)
+    assert "windows=4" in plan and "lines=4" in plan and "bins=fixed" in plan and "packed" in plan
+    assert "bins=float" in plan_ref
+    assert e < 1e-6
