@@ -243,6 +243,7 @@ struct bcsim_batch {
                 }
                 J.g.block = 1024;
                 J.g.chunks = t.chunk_order;
+                J.g.chunk_dist = t.chunk_dist;
                 J.nchunks = t.nchunks;
                 J.flags = (s->arc ? kJobArc : 0) | (cplx ? kJobPhase : 0) | (lut ? kJobLut : 0) |
                           (is_spline ? kJobSpline : 0) | (shared && !is_spline ? kJobShared : 0) |

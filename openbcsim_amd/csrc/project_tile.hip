@@ -247,8 +247,40 @@ __device__ __forceinline__ void tile_block(const ProjCommon& c, const CullParams
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
     const int nwaves = blockDim.x >> 6;
-    const int cbeg = slice * g.chunks_per_slice;
-    const int cend = min(nchunks, cbeg + g.chunks_per_slice);
+    int cbeg = slice * g.chunks_per_slice;
+    int cend = min(nchunks, cbeg + g.chunks_per_slice);
+    if (SHARED && ARC && g.chunk_dist && g.windows > 1) {
+        // the positions whose chunks can hold points at distances [r_lo, r_hi]: before the
+        // first far bound >= r_lo every chunk ends nearer, from the first near bound > r_hi on
+        // every chunk starts farther (a relative 1e-5 of slack for float rounding).  Wave 0
+        // finds the first, wave 1 the second, each by a 64-way search: 3 rounds of one load per
+        // lane for 2^18 chunks instead of 18 dependent loads.
+        __shared__ int win_range[2];
+        if (wave < 2) {
+            const float* bound = wave == 0 ? g.chunk_dist : g.chunk_dist + nchunks;
+            const float x = wave == 0 ? fmaxf(r_lo, 0.0f) * (1.0f - 1e-5f) : r_hi * (1.0f + 1e-5f) + 1e-30f;
+            // first position p in [0, nchunks) with bound[p] >= x (wave 0) or > x (wave 1)
+            int a = 0, b = nchunks;
+            while (a < b) {
+                const int stride = (b - a + 63) >> 6;
+                const int q = a + lane * stride;
+                bool before = false;
+                if (q < b) before = wave == 0 ? bound[q] < x : bound[q] <= x;  // a prefix of the lanes
+                const int cnt = __popcll(__ballot(before));
+                if (cnt == 0) {
+                    b = a;
+                } else {
+                    const int na = a + (cnt - 1) * stride + 1;
+                    b = min(b, a + cnt * stride);
+                    a = na;
+                }
+            }
+            if (lane == 0) win_range[wave] = a;
+        }
+        __syncthreads();
+        cbeg = max(cbeg, win_range[0]);
+        cend = min(cend, win_range[1]);
+    }
     const int N = SPLINE ? ss.n : fs.n;
 
     // The list is filled `step` chunks at a time and drained when another step

@@ -208,6 +208,11 @@ struct TileLaunch {
     const int* order;         // workgroup w runs block order[w] (heaviest first), or null: block w
     int* work;                // per-block hit counts added up by the launch, or null
     const int* chunks;        // chunk visiting order (a permutation of [0, nchunks)), or null: index order
+    // or null: for lines from a common origin, two ascending bounds per position p of the visiting
+    // order, far[p] (largest centre distance + box half-diagonal over positions <= p) then near[p]
+    // (smallest centre distance - half-diagonal over positions >= p); a radial window visits only
+    // the positions whose chunks can reach its distances
+    const float* chunk_dist;
     // or null: per launch, [0] profile-weight evaluations (LUT record gathers /
     // Gaussian exponentials), [1] LDS bin additions and [2] (chunk, line) pairs
     // the cull kept (each evaluated by a wave of 64 lanes), added up by the launch

@@ -79,18 +79,36 @@ struct DevBuf {
 // workgroup then sweeps the dataset from near to far at about the same pace, so
 // the workgroups sharing an L2 read nearby radial slices of the beam-profile
 // table at any one time.  The order never changes the sums.  Orders live as long as the dataset (at
-// most kMaxOrders origins; further origins visit the chunks in index order).
+// most kMaxOrders origins; further origins visit the chunks in index order).  An order from a
+// common origin is followed in the same buffer by two bounds per position p of the order, the
+// largest far distance (centre distance + box half-diagonal) over positions <= p and the smallest
+// near distance (centre distance - half-diagonal) over positions >= p; both ascend with p, so a
+// radial window of the tile kernel finds by two binary searches the positions whose chunks can
+// reach its distances.  A chunk with a large box (where the z-curve jumps) widens only the
+// windows near its own distance.
 struct ChunkOrders {
     using Key = std::array<float, 7>;  // origin, direction (depth mode), shared-origin flag
     static constexpr size_t kMaxOrders = 16;
     std::vector<float4> centres;       // box centres; w = 1 marks an empty chunk
+    std::vector<float> half_diag;      // box half-diagonals
     std::mutex mu;
     std::vector<std::pair<Key, std::unique_ptr<DevBuf>>> orders;
 
     void set_boxes(const std::vector<ChunkBox>& boxes) {
         centres.resize(boxes.size());
-        for (size_t c = 0; c < boxes.size(); c++)
-            centres[c] = make_float4(boxes[c].cx, boxes[c].cy, boxes[c].cz, boxes[c].hx >= 0.0f ? 0.0f : 1.0f);
+        half_diag.resize(boxes.size());
+        for (size_t c = 0; c < boxes.size(); c++) {
+            const ChunkBox& b = boxes[c];
+            centres[c] = make_float4(b.cx, b.cy, b.cz, b.hx >= 0.0f ? 0.0f : 1.0f);
+            half_diag[c] = b.hx >= 0.0f ? (float)(std::sqrt((double)b.hx * b.hx + (double)b.hy * b.hy +
+                                                            (double)b.hz * b.hz) * (1.0 + 1e-6))
+                                        : 0.0f;
+        }
+    }
+
+    // The bounds that follow a common-origin order of n chunks: far[n] then near[n], or null.
+    static const float* distances(const Key& key, const int* order, int n) {
+        return key[6] != 0.0f && order ? reinterpret_cast<const float*>(order + n) : nullptr;
     }
 
     // Device array of chunk indices (a permutation of [0, nchunks)), or null.
@@ -109,8 +127,24 @@ struct ChunkOrders {
             k[c] = {v, (int)c};
         }
         std::sort(k.begin(), k.end());
-        std::vector<int> order(k.size());
-        for (size_t c = 0; c < k.size(); c++) order[c] = k[c].second;
+        const bool shared = key[6] != 0.0f;
+        const size_t n = k.size();
+        std::vector<int> order(n * (shared ? 3 : 1));
+        for (size_t c = 0; c < n; c++) order[c] = k[c].second;
+        if (shared) {
+            // running max of far distances from the front, running min of near ones from the back
+            float far = -INFINITY, near = INFINITY;
+            for (size_t c = 0; c < n; c++) {
+                const float d = std::sqrt(k[c].first), h = half_diag[k[c].second];
+                far = std::max(far, d + h);  // infinite from the first empty chunk on
+                std::memcpy(&order[n + c], &far, sizeof far);
+            }
+            for (size_t c = n; c-- > 0;) {
+                const float d = std::sqrt(k[c].first), h = half_diag[k[c].second];
+                near = std::min(near, d - h);
+                std::memcpy(&order[2 * n + c], &near, sizeof near);
+            }
+        }
         auto buf = std::make_unique<DevBuf>();
         buf->ensure(order.size() * sizeof(int));
         hip_check(hipMemcpy(buf->p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
@@ -1366,6 +1400,7 @@ struct bcsim_sim {
         const SplineDataset* sp = nullptr;  // spline kinds
         SplineScan* scan = nullptr;
         const int* chunk_order = nullptr;   // visiting order of the chunks (ChunkOrders), or null
+        const float* chunk_dist = nullptr;  // their reach from the common origin (ChunkOrders), or null
     };
 
     // Largest |weight| of the beam profile: 1 for the Gaussian, the largest |sample| of a LUT.
@@ -1428,7 +1463,10 @@ struct bcsim_sim {
             t.boxes = f->boxes.as<ChunkBox>();
             t.nchunks = f->nchunks;
             t.n = f->n;
-            if (chunk_order) t.chunk_order = f->visit.get(okey, create);
+            if (chunk_order) {
+                t.chunk_order = f->visit.get(okey, create);
+                t.chunk_dist = ChunkOrders::distances(okey, t.chunk_order, t.nchunks);
+            }
             out.push_back(t);
         }
         for (size_t q = 0; q < splines.size(); q++) {
@@ -1455,7 +1493,10 @@ struct bcsim_sim {
             t.n = sp->n;
             t.sp = sp;
             t.scan = scan;
-            if (chunk_order) t.chunk_order = sp->visit.get(okey, create);
+            if (chunk_order) {
+                t.chunk_order = sp->visit.get(okey, create);
+                t.chunk_dist = ChunkOrders::distances(okey, t.chunk_order, t.nchunks);
+            }
             out.push_back(t);
         }
         return out;
@@ -1506,6 +1547,7 @@ struct bcsim_sim {
         const bool is_spline = std::strcmp(t.kind, "spline") == 0;
         TileLaunch cfg = cfg_in;
         cfg.chunks = t.chunk_order;
+        cfg.chunk_dist = t.chunk_dist;
         hip_check(launch_project_tile(t.c, cull_params(t.c), is_spline ? nullptr : &t.fs, is_spline ? &t.ss : nullptr,
                                       t.boxes, t.nchunks, bins, arc, phase_delay, profile == kProfileLut,
                                       shared_origin() && !is_spline, cfg, stream),

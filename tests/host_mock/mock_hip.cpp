@@ -497,6 +497,18 @@ void check_tile(const ProjCommon& c, const FixedSet* fs, const SplineSet* ss, co
             seen[ci] = 1;
         }
     }
+    if (g.chunk_dist) {
+        // the far and near bounds of the visiting order: ascending, after the order in its buffer
+        if (!g.chunks || g.chunk_dist != reinterpret_cast<const float*>(g.chunks + nchunks))
+            die(std::string(op) + ": chunk distances without their order");
+        check(g.chunk_dist, (size_t)nchunks * 8, "chunk distances", op);
+        for (int i = 1; i < 2 * nchunks; i++)
+            if (i != nchunks && !(g.chunk_dist[i - 1] <= g.chunk_dist[i]))
+                die(std::string(op) + ": chunk distance bounds not ascending");
+        for (int i = 0; i < nchunks; i++)
+            if (!(g.chunk_dist[nchunks + i] <= g.chunk_dist[i]) && std::isfinite(g.chunk_dist[i]))
+                die(std::string(op) + ": near bound above far bound");
+    }
     const int blocks = ((c.num_lines + g.tile_lines - 1) / g.tile_lines) * g.windows;
     if (g.work) check(g.work, (size_t)blocks * 4, "work", op);
     if (g.order) {

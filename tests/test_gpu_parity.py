@@ -518,3 +518,47 @@ def test_linear_scan_with_lut(arc):
     e = rel_l2(iq, ref)
     print(f"[result] linear scan with LUT arc={arc}: rel L2 vs oracle {e:.2e}", flush=True)
     assert np.abs(ref).max() > 0 and e <= 1e-4
+
+
+def test_radial_windows_visit_every_reachable_chunk():
+    """Radial windows visit only the chunks whose boxes can reach their distances
+    (bounds kept with each common-origin chunk order).  A phantom of three dense
+    clusters at different depths plus a sparse background has chunks whose boxes
+    span the gaps where the z-curve jumps; frames in 2 and 4 windows must equal the
+    single-window frame, and the oracle on sampled lines."""
+    from openbcsim_amd.pyrfsim import RfSimulator
+    rng = np.random.default_rng(47)
+    parts = []
+    for centre, half, count in (((0.0, 0.0, 0.02), 0.004, 60000), ((0.01, -0.005, 0.07), 0.006, 60000),
+                                ((-0.008, 0.006, 0.12), 0.005, 60000), ((0.0, 0.0, 0.075), 0.03, 20000)):
+        p = np.empty((count, 4), np.float32)
+        p[:, :3] = rng.uniform(-half, half, (count, 3)) + np.array(centre)
+        p[:, 3] = rng.standard_normal(count)
+        parts.append(p)
+    wl = W.volume_lut(1000, n_az=32, n_el=32, seed=47)
+    wl.fixed = [np.concatenate(parts)]
+    frames, plans = {}, {}
+    for windows in (1, 2, 4):
+        g = RfSimulator("gpu")
+        wl.apply(g)
+        g.set_parameter("tile_windows", str(windows))
+        g.set_parameter("tile_lines", "4")
+        frames[windows] = g.simulate_frame()
+        plans[windows] = g.get_parameter("launch_plan").splitlines()[0]
+        g.close()
+    for windows in (2, 4):
+        e = rel_l2(frames[windows], frames[1])
+        print(f"[result] {plans[windows]!r}: vs one window rel_l2={e:.2e}", flush=True)
+        assert f"windows={windows}" in plans[windows]
+        assert e < 1e-6
+    pick = np.array([0, 200, 527, 1023])
+    o_, d, l, t, length = wl.scan
+    sub = W.volume_lut(1000, n_az=32, n_el=32, seed=47)
+    sub.fixed = wl.fixed
+    sub.scan = (o_[pick], d[pick], l[pick], t[pick], length)
+    o = Oracle()
+    sub.apply(o)
+    o.set_parameter("weight_cutoff", "1e-8")
+    err = rel_l2(frames[4][pick], o.simulate_frame())
+    print(f"[result] 4 windows vs oracle (4 lines) rel_l2={err:.2e}", flush=True)
+    assert err <= TOL
