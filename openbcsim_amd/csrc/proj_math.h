@@ -45,9 +45,11 @@ __device__ __forceinline__ float gauss_weight(const ProfileParams& P, float l, f
 
 // LUTBeamProfile::sampleProfile (BeamProfile.cpp:79-130): truncating indices, zero
 // whenever a neighbour falls outside the table, trilinear in r, then l, then e.
-// The table is read as quad pairs: quad[2*((r*nl + l)*ne + e) + s] holds the
+// The table is read as quad pairs: quad[2*((r*ne + e)*nl + l) + s] holds the
 // samples at (l, e), (l, e+1), (l+1, e), (l+1, e+1) of radial slice r + s, so the
 // eight corners are two 16-byte loads from one 32-byte record (one cache line).
+// The lateral index runs fastest: lines next to each other in azimuth look up a
+// point a few lateral cells apart, often in the same 128-byte cache line.
 // Upper neighbour = lower + 1, except that a coordinate
 // in (-1, 0) truncates to 0 for both (the reference's int(t) and int(t + 1)).
 // The radial part depends on r only, so a caller projecting one point onto
@@ -88,7 +90,7 @@ __device__ __forceinline__ float lut_weight_le(const ProfileParams& P, const Lut
     if (P.slice_box && (l0 < R.box.x || l0 > R.box.y || e0 < R.box.z || e0 > R.box.w)) return 0.0f;
     t.evals++;
     const float fl = tl - (float)l0, fe = te - (float)e0;
-    const float4* cell = P.quad + 2 * (R.off + l0 * P.ne + e0);
+    const float4* cell = P.quad + 2 * (R.off + e0 * P.nl + l0);
     float4 a = cell[0];
     float4 b = cell[1];
     if (R.low) b = a;

@@ -55,7 +55,7 @@ struct ProfileParams {
     float inv_tsl, inv_tse;
     float g_cut;
     // LUT (BeamProfile.cpp:79-130): t = (x - x_min) * inv_dx, trilinear, zero outside;
-    // read as quad pairs (see lut_weight): quad[2*((r*nl + l)*ne + e) + s] holds slice r + s
+    // read as quad pairs (see lut_weight): quad[2*((r*ne + e)*nl + l) + s] holds slice r + s
     const float4* quad;
     int nr, nl, ne;
     float r_min, l_min, e_min;

@@ -449,7 +449,7 @@ struct bcsim_sim {
     // profile
     int profile = -1;
     float sigma_l = 0.f, sigma_e = 0.f;
-    std::shared_ptr<DevBuf> lut;  // float4 quad pairs [n_rad][n_lat][n_ele][2]; replaced, never rewritten
+    std::shared_ptr<DevBuf> lut;  // float4 quad pairs [n_rad][n_ele][n_lat][2]; replaced, never rewritten
     std::shared_ptr<const std::vector<float>> lut_samples;  // the table as given, for the cutoff's boxes
     float lut_peak = 0.0f;                                   // largest |sample|
     std::shared_ptr<DevBuf> lut_boxes;                       // ProfileParams::slice_box for lut_boxes_cut
@@ -927,7 +927,7 @@ struct bcsim_sim {
         for (int r = 0; r < nr; r++)
             for (int l = 0; l < nl; l++)
                 for (int e = 0; e < ne; e++) {
-                    const size_t cell = ((size_t)r * nl + l) * ne + e;
+                    const size_t cell = ((size_t)r * ne + e) * nl + l;
                     quad[2 * cell] = q(r, l, e);
                     quad[2 * cell + 1] = q(r + 1 < nr ? r + 1 : r, l, e);
                 }
